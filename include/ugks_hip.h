@@ -1,0 +1,187 @@
+/*
+ * ugks_hip.h -- C ABI of libugks_hip.so, the MI355X (gfx950) UGKS time step.
+ *
+ * Two layers share one block descriptor:
+ *
+ *  1. Reference-shaped entry points, one per OpenCL kernel of pyUGKS's time
+ *     step.  Each keeps the kernel's name (prefixed ugks_) and argument
+ *     order; the block handle and a HIP stream replace the pyopencl queue and
+ *     NDRange (the launch shape is derived from the block and the face/offset
+ *     arguments, as ugks_block.py derives it).  Device buffers are plain
+ *     device pointers in the reference's layout: f-like arrays are
+ *     [NI][NJ][NV] pairs (g,h) of double, face/cell arrays [NI][NJ] double4,
+ *     macro [ni][nj] double4 etc. (ugks_block.py:212-255, :338-407).
+ *
+ *  2. Fused entry points (ugks_step_*): one block step in a few launches with
+ *     the same buffers, plus the mixed FP32/FP64 and batched variants.
+ *
+ * Every function returns 0 on success, a negative value on failure
+ * (-1: bad argument, -2: unsupported configuration, otherwise -(hipError_t));
+ * ugks_last_error() describes the last failure of the calling thread.
+ * Errors detected by kernels travel through the reference's flag buffer
+ * (flag[0]: 1 = NaN in f, 4 = adsorption isotherm clamp; clError.c:4-13).
+ * Calls are stream-ordered; one block must not be used from two streams at
+ * once.
+ */
+#ifndef UGKS_HIP_H
+#define UGKS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Per-block constants: the #define set genHeader bakes per block
+ * (ugks_CL.py:58-395), held at run time instead of compiled in. */
+typedef struct ugks_block_desc {
+    int32_t ni, nj, NI, NJ, NV, ghost;
+    int32_t flux_method;            /* 0 vanLeer, 1 WENO5, 3 minmod (ugks_CL.py:33-44) */
+    int32_t relax_type;             /* clFunctions.c:8-32 */
+    double Pr, Kn_eff, gam, B, K, omega, CFL, umax, vmax;
+    double wall_D[4], wall_U[4], wall_V[4], wall_T[4], wall_P[4];  /* N,E,S,W; T is 1/T */
+    double beta_n[4], beta_t[4], gamma_f[4], alpha_p[4];           /* ugks_CL.py:102-132 */
+    int32_t dynamic_vartheta[4], constant_adsorb[4];
+    double adsorb[4];                                              /* ugks_CL.py:163-167 */
+    const double *quad;             /* host [NV][2]  (QUAD,   ugks_CL.py:318-333) */
+    const double *weight;           /* host [NV]     (WEIGHT, ugks_CL.py:335-346) */
+    const int64_t *mirror_NS, *mirror_EW, *mirror_D;  /* host [NV] (ugks_CL.py:348-395) */
+} ugks_block_desc;
+
+typedef struct ugks_block ugks_block;
+
+/* ---- runtime ----------------------------------------------------------- */
+/* Replaces UGKSBlock.block_CL/update_CL (ugks_block.py:151-189): copies the
+ * velocity tables to the device once; no per-block compilation. */
+int ugks_block_create(const ugks_block_desc *desc, ugks_block **out);
+int ugks_block_destroy(ugks_block *blk);
+const char *ugks_last_error(void);
+int ugks_get_device_count(int *count);
+int ugks_set_device(int device);
+int ugks_malloc(void **ptr, size_t bytes);
+int ugks_free(void *ptr);
+int ugks_memset(void *ptr, int value, size_t bytes, void *stream);
+int ugks_memcpy_htod(void *dst, const void *src, size_t bytes, void *stream);
+int ugks_memcpy_dtoh(void *dst, const void *src, size_t bytes, void *stream);
+int ugks_memcpy_dtod(void *dst, const void *src, size_t bytes, void *stream);
+int ugks_stream_create(void **stream);
+int ugks_stream_destroy(void *stream);
+int ugks_stream_synchronize(void *stream);
+int ugks_event_create(void **event);
+int ugks_event_destroy(void *event);
+int ugks_event_record(void *event, void *stream);
+int ugks_event_elapsed_ms(float *ms, void *start, void *stop);
+
+/* ---- reference-shaped kernels (layer 1) -------------------------------- */
+/* clKernel.c:31-37, launched (Ni,Nj) at ugks_block.py:748-751 */
+int ugks_cellGeom(ugks_block *blk, void *stream, const void *xy, void *area, void *centre,
+                  void *mid_side, void *normal, void *side_length);
+/* clKernel.c:141-144 (constant WALL_* values), ugks_block.py:603-619 */
+int ugks_paraBC(ugks_block *blk, void *stream, const void *para_def, void *wall_prop, double t);
+/* clKernel.c:224-227, ugks_block.py:1420-1421 */
+int ugks_clFindDT(ugks_block *blk, void *stream, const void *xy, const void *area,
+                  const void *macro, void *time_step, double Kn);
+/* device max of time_step into out[0] (replaces cl_array.max, ugks_block.py:1426) */
+int ugks_time_step_max(ugks_block *blk, void *stream, const void *time_step, void *out);
+/* clKernel.c:382-384, ugks_block.py:413-415 */
+int ugks_initFunctions(ugks_block *blk, void *stream, void *Fin, const void *macro, const void *gQ);
+/* clKernel.c:414-416, ugks_block.py:419-420 */
+int ugks_calcMacro(ugks_block *blk, void *stream, const void *Fin, void *macro);
+/* clKernel.c:480-483, ugks_block.py:1271-1272 */
+int ugks_calcQ(ugks_block *blk, void *stream, const void *Fin, const void *macro, void *gQ);
+/* clUGKS.c:110-117, ugks_block.py:1066-1068 */
+int ugks_iFace(ugks_block *blk, void *stream, const void *Fin, void *iface_f, void *fsigma,
+               const void *centre, const void *mid_side, const void *normal, int32_t face);
+/* clUGKS.c:209-214, ugks_block.py:1107-1109 */
+int ugks_getFaceCons(ugks_block *blk, void *stream, const void *iface_f, const void *normal,
+                     int32_t face, void *primary, int32_t offset_bottom, int32_t offset_top);
+/* clUGKS.c:273-278, ugks_block.py:1111-1113 */
+int ugks_getAL(ugks_block *blk, void *stream, const void *Fin, const void *normal, int32_t face,
+               void *gaL, int32_t offset_bottom, int32_t offset_top);
+/* clUGKS.c:337-342, ugks_block.py:1115-1117 */
+int ugks_getAR(ugks_block *blk, void *stream, const void *Fin, const void *normal, int32_t face,
+               void *gaR, int32_t offset_bottom, int32_t offset_top);
+/* clUGKS.c:401-408, ugks_block.py:1123-1125 */
+int ugks_getPLR(ugks_block *blk, void *stream, const void *centre, const void *mid_side,
+                int32_t face, void *primary, void *gaL, void *gaR, int32_t offset_bottom,
+                int32_t offset_top);
+/* clUGKS.c:446-454, ugks_block.py:1129-1132 */
+int ugks_initMacroFlux(ugks_block *blk, void *stream, void *flux_macro, int32_t face, double dt,
+                       const void *primary, const void *gaL, const void *gaR, void *gaT,
+                       void *gMxi, int32_t offset_bottom, int32_t offset_top);
+/* clUGKS.c:522-527, ugks_block.py:1138-1140 */
+int ugks_calcFaceQ(ugks_block *blk, void *stream, const void *iface_f, const void *primary,
+                   const void *normal, void *faceQ, int32_t face, int32_t offset_bottom,
+                   int32_t offset_top);
+/* clUGKS.c:587-596, ugks_block.py:1145-1148 */
+int ugks_macroFlux(ugks_block *blk, void *stream, const void *flux_f, const void *fsigma,
+                   void *flux_macro, const void *normal, const void *side_length, int32_t face,
+                   double dt, const void *primary, const void *faceQ, int32_t offset_bottom,
+                   int32_t offset_top);
+/* clUGKS.c:683-695, ugks_block.py:1150-1154 */
+int ugks_distFlux(ugks_block *blk, void *stream, void *flux_f, const void *fsigma,
+                  const void *normal, const void *side_length, int32_t face, double dt,
+                  const void *primary, const void *gaL, const void *gaR, const void *gaT,
+                  const void *gMxi, const void *faceQ, int32_t offset_bottom, int32_t offset_top);
+/* clUGKS.c:776-781, ugks_block.py:936-938 */
+int ugks_accommodatingWallDist(ugks_block *blk, void *stream, const void *normal, int32_t face,
+                               const void *wall_prop, void *flux_f, double dt);
+/* clUGKS.c:911-915, ugks_block.py:973-975 */
+int ugks_wallFlux(ugks_block *blk, void *stream, const void *normal, const void *side_length,
+                  int32_t face, void *flux_f, void *flux_macro, double dt);
+/* clUGKS.c:1019-1026, ugks_block.py:979-981 */
+int ugks_wallMassEnergyFluxes(ugks_block *blk, void *stream, const void *normal,
+                              const void *side_length, const void *area, int32_t face,
+                              const void *flux_f, void *wall_flux, double dt);
+/* clAdsorb.c:253-262, ugks_block.py:942-945 */
+int ugks_adsorbingWall_P1(ugks_block *blk, void *stream, const void *normal, int32_t face,
+                          const void *wall_prop, void *wall_cover, void *wall_dist,
+                          const void *flux_f, const void *macro, double dt, void *flag);
+/* clAdsorb.c:656-664, ugks_block.py:966-969 */
+int ugks_adsorbingWallS_P2(ugks_block *blk, void *stream, const void *normal, int32_t face,
+                           const void *wall_prop, const void *wall_cover, const void *wall_dist,
+                           void *flux_f, const void *macro, double dt);
+/* clAdsorb.c:821-829, ugks_block.py:960-963 */
+int ugks_adsorbingWallD_P2(ugks_block *blk, void *stream, const void *normal, int32_t face,
+                           const void *wall_prop, const void *wall_cover, const void *wall_dist,
+                           void *flux_f, const void *macro, double dt);
+/* clUGKS.c:1158-1163, ugks_block.py:1279-1281 */
+int ugks_updateMacro(ugks_block *blk, void *stream, const void *flux_macro_S,
+                     const void *flux_macro_W, const void *area, void *macro, void *residual);
+/* clUGKS.c:1189-1197, ugks_block.py:1292-1295 */
+int ugks_UGKS_update(ugks_block *blk, void *stream, void *Fin, const void *flux_f_S,
+                     const void *flux_f_W, const void *area, const void *macro, const void *gQ,
+                     const void *residual, double dt, void *flag);
+/* clUGKS.c:1259-1261, ugks_block.py:1311-1312 */
+int ugks_getResidual(ugks_block *blk, void *stream, const void *macro, void *residual);
+/* clUGKS.c:1282-1287, ugks_block.py:910-911 */
+int ugks_edgeMassFlow(ugks_block *blk, void *stream, const void *macro, const void *normal,
+                      const void *side_length, void *mdot, int32_t this_face);
+/* clGhostRoutines.c:407-414, ugks_block.py:468-470 (blkB supplies NV and the mirror tables
+ * are taken from blk; NIB/NJB as in the reference) */
+int ugks_edgeExchange(ugks_block *blk, void *stream, void *fA, const void *xyA, int32_t this_face,
+                      const void *fB, const void *xyB, int32_t NIB, int32_t NJB,
+                      int32_t that_face, int32_t ori, int32_t flip);
+/* clGhostRoutines.c:544-546, ugks_block.py:491-492 */
+int ugks_edgeExtrapolate(ugks_block *blk, void *stream, void *Fin, int32_t this_face);
+/* clGhostRoutines.c:603-606, ugks_block.py:560-561 */
+int ugks_edgeConstant(ugks_block *blk, void *stream, void *Fin, int32_t this_face,
+                      const void *wall_prop);
+/* clGhostRoutines.c:664-670, ugks_block.py:512-514 */
+int ugks_edgeInflow(ugks_block *blk, void *stream, void *Fin, int32_t this_face,
+                    const void *normal, const void *macro, const void *gQ, const void *wall_prop);
+/* clGhostRoutines.c:755-761, ugks_block.py:537-539 */
+int ugks_edgeOutflow(ugks_block *blk, void *stream, void *Fin, int32_t this_face,
+                     const void *normal, const void *macro, const void *gQ, const void *wall_prop);
+/* clGhostRoutines.c:845-847, ugks_block.py:582 */
+int ugks_edgeMirror(ugks_block *blk, void *stream, void *Fin, int32_t this_face);
+/* clGhostRoutines.c:911-913, ugks_block.py:580 */
+int ugks_edgeBounceBack(ugks_block *blk, void *stream, void *Fin, int32_t this_face);
+/* clGhostRoutines.c:967-968, ugks_block.py:600-601 */
+int ugks_edgeConstGrad(ugks_block *blk, void *stream, void *Fin, int32_t this_face);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

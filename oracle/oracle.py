@@ -1,0 +1,234 @@
+"""ctypes driver of the FP64 CPU oracle (oracle/ugks_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke()
+and by the cpu_baseline leg of bench.py, as the checker.  The product
+(pyugks_amd) never imports it.
+
+``OracleSim(gdata)`` builds the same blocks as ``pyugks_amd.sim.UGKSim``
+from the same case definition, holds every buffer as a numpy array in the
+reference layout, and runs the reference sequence in C.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "_build", "liboracle.so")
+SRC = os.path.join(HERE, "ugks_oracle.c")
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle with gcc (OpenMP, no FP contraction)."""
+    os.makedirs(os.path.dirname(LIB), exist_ok=True)
+    if force or not os.path.exists(LIB) or os.path.getmtime(LIB) < os.path.getmtime(SRC):
+        tmp = LIB + ".tmp"
+        subprocess.run(["gcc", "-O2", "-std=c11", "-fopenmp", "-fPIC", "-shared", "-ffp-contract=off",
+                        "-o", tmp, SRC, "-lm"], check=True)
+        os.replace(tmp, LIB)
+    return LIB
+
+
+P = C.c_void_p
+I = C.c_int32
+D = C.c_double
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("ni", I), ("nj", I), ("NI", I), ("NJ", I), ("NV", I), ("ghost", I), ("flux_method", I), ("relax_type", I),
+        ("Pr", D), ("Kn_eff", D), ("gam", D), ("B", D), ("K", D), ("omega", D), ("CFL", D), ("umax", D), ("vmax", D),
+        ("wall_D", D * 4), ("wall_U", D * 4), ("wall_V", D * 4), ("wall_T", D * 4), ("wall_P", D * 4),
+        ("beta_n", D * 4), ("beta_t", D * 4), ("gamma_f", D * 4), ("alpha_p", D * 4),
+        ("dynamic_vartheta", I * 4), ("constant_adsorb", I * 4), ("adsorb", D * 4),
+        ("quad", P), ("weight", P), ("mirror_NS", P), ("mirror_EW", P), ("mirror_D", P),
+    ]
+
+
+STATE_FIELDS = ["f", "flux_f_S", "flux_f_W", "sigma", "macro", "Q", "flux_macro_S", "flux_macro_W", "prim", "aL",
+                "aR", "aT", "faceQ", "Mxi", "xy", "area", "centre", "mid_side", "normal", "side_length", "wall_prop",
+                "wall_cover", "wall_fluxes", "wall_dist", "time_step", "residual", "bc_mdot", "flag"]
+
+
+class State(C.Structure):
+    _fields_ = [(n, P) for n in STATE_FIELDS]
+
+
+class BC(C.Structure):
+    _fields_ = [("type", I), ("other_block", I), ("other_face", I), ("orientation", I), ("flip", I),
+                ("reflect_type", I)]
+
+
+class Block(C.Structure):
+    _fields_ = [("p", Params), ("s", State), ("bc", BC * 4)]
+
+
+class Sim(C.Structure):
+    _fields_ = [("nblocks", I), ("blocks", C.POINTER(Block)), ("dt", D), ("time", D), ("CFL", D),
+                ("delta_dt", D), ("suggest_dt", D), ("Kn_eff", D), ("step", I), ("force_time", I),
+                ("dt_update_count", I), ("max_dt", P)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        _lib = C.CDLL(build())
+        _lib.or_sim_init.argtypes = [C.POINTER(Sim)]
+        _lib.or_sim_one_step.argtypes = [C.POINTER(Sim), C.c_int]
+        _lib.or_sim_run.argtypes = [C.POINTER(Sim), C.c_int]
+        _lib.or_set_threads.argtypes = [C.c_int]
+        for n in ("or_sim_init", "or_sim_one_step", "or_sim_run"):
+            getattr(_lib, n).restype = C.c_int
+    return _lib
+
+
+def set_threads(n: int):
+    lib().or_set_threads(int(n))
+
+
+class V2(C.Structure):
+    _fields_ = [("x", D), ("y", D)]
+
+
+class V4(C.Structure):
+    _fields_ = [("s0", D), ("s1", D), ("s2", D), ("s3", D)]
+
+
+def helpers():
+    """Bind the oracle's scalar helpers (for checks against closed forms)."""
+    L = lib()
+    PP = C.POINTER(Params)
+    sig = {
+        "or_relaxTime": (D, [PP, V4]),
+        "or_fM": (V2, [PP, V4, V2, C.c_int]),
+        "or_fS": (V2, [PP, V4, V2, V2, V2]),
+        "or_momentU": (None, [PP, V4, C.POINTER(D), C.POINTER(D), C.POINTER(D), C.POINTER(D), C.POINTER(D)]),
+        "or_timeIntegrals": (None, [D, D, C.POINTER(D)]),
+        "or_vanLeer": (V2, [V2, V2]),
+        "or_WENO5": (V2, [C.POINTER(V2), D]),
+        "or_NND": (V2, [C.POINTER(V2), D]),
+        "or_microSlope": (V4, [PP, V4, V4]),
+        "or_getPrimary": (V4, [PP, V4]),
+        "or_getConserved": (V4, [PP, V4]),
+    }
+    for n, (r, a) in sig.items():
+        f = getattr(L, n)
+        f.restype = r
+        f.argtypes = a
+    return L
+
+
+def make_params(consts: dict, tables) -> Params:
+    q, w, mns, mew, md = tables
+    p = Params()
+    for k in ("ni", "nj", "NI", "NJ", "NV", "ghost", "flux_method", "relax_type"):
+        setattr(p, k, int(consts[k]))
+    for k in ("Pr", "Kn_eff", "gam", "B", "K", "omega", "CFL", "umax", "vmax"):
+        setattr(p, k, float(consts[k]))
+    for k in ("wall_D", "wall_U", "wall_V", "wall_T", "wall_P", "beta_n", "beta_t", "gamma_f", "alpha_p", "adsorb"):
+        setattr(p, k, (D * 4)(*[float(x) for x in consts[k]]))
+    for k in ("dynamic_vartheta", "constant_adsorb"):
+        setattr(p, k, (I * 4)(*[int(x) for x in consts[k]]))
+    p.quad, p.weight = q.ctypes.data, w.ctypes.data
+    p.mirror_NS, p.mirror_EW, p.mirror_D = mns.ctypes.data, mew.ctypes.data, md.ctypes.data
+    return p
+
+
+class OracleBlock:
+    def __init__(self, gdata, bdef, xy):
+        from pyugks_amd.data import initial_macro
+        self.consts = gdata.block_constants(bdef)
+        self.tables = gdata.velocity_tables()
+        c = self.consts
+        NI, NJ, NV, ni, nj = c["NI"], c["NJ"], c["NV"], c["ni"], c["nj"]
+        wl = max(ni, nj)
+        z = np.zeros
+        self.arrays = dict(
+            f=z((NI, NJ, NV, 2)), flux_f_S=z((NI, NJ, NV, 2)), flux_f_W=z((NI, NJ, NV, 2)),
+            sigma=z((NI, NJ, NV, 2)), macro=z((ni, nj, 4)), Q=z((ni, nj, 2)), flux_macro_S=z((NI, NJ, 4)),
+            flux_macro_W=z((NI, NJ, 4)), prim=z((NI, NJ, 4)), aL=z((NI, NJ, 4)), aR=z((NI, NJ, 4)),
+            aT=z((NI, NJ, 4)), faceQ=z((NI, NJ, 2)), Mxi=z((NI, NJ, 3)),
+            xy=np.ascontiguousarray(xy, dtype=np.float64), area=np.full((NI, NJ), -1.0),
+            centre=np.full((NI, NJ, 2), -1.0), mid_side=np.full((NI, NJ, 2, 2), -1.0),
+            normal=np.full((NI, NJ, 2, 2), -1.0), side_length=np.full((NI, NJ, 2), -1.0),
+            wall_prop=z((wl, 4, 4)), wall_cover=z((wl, 4, 4)), wall_fluxes=z((wl, 4, 8)),
+            wall_dist=z((wl, NV, 2)), time_step=z((ni, nj)), residual=z((ni, nj, 4)), bc_mdot=z((wl, 4)),
+            flag=z(2))
+        m, q = initial_macro(bdef)
+        self.arrays["macro"][:] = m
+        self.arrays["Q"][:] = q
+        for bci, bcx in enumerate(bdef.bc_list):
+            self.arrays["wall_cover"][:, bci, 0] = bcx.cover_initial
+        self.block = Block()
+        self.block.p = make_params(c, self.tables)
+        st = State()
+        for n in STATE_FIELDS:
+            setattr(st, n, self.arrays[n].ctypes.data)
+        self.block.s = st
+        rt = {"S": 0, "D": 1, "CL": 2}
+        for k, bcx in enumerate(bdef.bc_list):
+            self.block.bc[k] = BC(int(bcx.type_of_BC), int(bcx.other_block), int(bcx.other_face),
+                                  int(bcx.orientation), int(bcx.flip_distribution), rt.get(bcx.reflect_type, 0))
+
+    def __getitem__(self, name):
+        return self.arrays[name]
+
+
+class OracleSim:
+    """The reference sequence (UGKSim.__init__ step part, one_step, run) in C."""
+
+    def __init__(self, gdata, prepared: bool = False, threads: int | None = None):
+        from pyugks_amd.data import ghost_coordinates
+        if not prepared:
+            gdata.prepare()
+        self.gdata = gdata
+        if threads is not None:
+            set_threads(threads)
+        xys = ghost_coordinates(gdata.blocks, gdata.ghost)
+        self.blocks = [OracleBlock(gdata, b, xys[i]) for i, b in enumerate(gdata.blocks)]
+        n = len(self.blocks)
+        self._barr = (Block * n)(*[b.block for b in self.blocks])
+        self.max_dt = np.zeros(n)
+        self.sim = Sim()
+        self.sim.nblocks = n
+        self.sim.blocks = C.cast(self._barr, C.POINTER(Block))
+        self.sim.dt = gdata.dt
+        self.sim.time = gdata.time
+        self.sim.CFL = gdata.CFL
+        self.sim.delta_dt = gdata.delta_dt
+        self.sim.suggest_dt = gdata.suggest_dt if gdata.suggest_dt else 0.0
+        self.sim.Kn_eff = gdata.Kn_eff
+        self.sim.step = 0
+        self.sim.force_time = 0
+        self.sim.dt_update_count = gdata.dt_update_count
+        self.sim.max_dt = self.max_dt.ctypes.data
+        rc = lib().or_sim_init(C.byref(self.sim))
+        if rc:
+            raise RuntimeError(f"oracle init failed {rc}")
+
+    def one_step(self, get_dt=True):
+        rc = lib().or_sim_one_step(C.byref(self.sim), int(bool(get_dt)))
+        if rc:
+            raise RuntimeError(f"oracle step failed {rc}")
+
+    def run(self, nsteps: int):
+        rc = lib().or_sim_run(C.byref(self.sim), int(nsteps))
+        if rc:
+            raise RuntimeError(f"oracle run failed {rc}")
+
+    @property
+    def dt(self):
+        return self.sim.dt
+
+    @property
+    def time(self):
+        return self.sim.time
+
+    def macro(self):
+        return [b["macro"].copy() for b in self.blocks]

@@ -1,0 +1,281 @@
+"""Host counterpart of ugks_block.UGKSBlock, driving libugks_hip.so.
+
+Method names, buffer names and call order follow ugks_block.py; the
+pyopencl launches ``self.prg.<kernel>(queue, global, local, *args)`` become
+``self.prg.<kernel>(stream, *args)`` on the HIP kernels of the same name
+(launch shapes are derived inside the library).  All buffers are FP64 device
+arrays in the reference layout (ugks_block.py:212-255, :338-407).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import bc as bcd
+from . import lib as L
+from .data import UGKSData, BlockDef, initial_macro
+
+SOUTH, WEST = np.int32(0), np.int32(1)
+
+
+class UGKSBlock:
+    """One block of the simulation (ugks_block.py:46-1623, step-relevant part)."""
+
+    def __init__(self, block_id: int, gdata: UGKSData, bdef: BlockDef, xy: np.ndarray, stream=None):
+        self.id = block_id
+        self.gdata = gdata
+        self.bdef = bdef
+        self.stream = stream
+        self.ghost = gdata.ghost
+        self.ni, self.nj = bdef.ni, bdef.nj
+        self.Nv = gdata.Nv
+        self.Ni = self.ni + 2 * self.ghost
+        self.Nj = self.nj + 2 * self.ghost
+        self.bc_list = bdef.bc_list
+        self.fill_condition = bdef.fill_condition
+        self.residual = 1e10
+        self.max_dt = 0.0
+        self.has_accommodating = False
+        self.blockList = []
+        # block_CL / update_CL (ugks_block.py:151-189): constants + tables -> device
+        self.consts = gdata.block_constants(bdef)
+        desc, keep = L.make_desc(self.consts, gdata.velocity_tables())
+        self.prg = L.Program(desc, keep)
+        self.initGeom(xy)
+
+    # ------------------------------------------------------------ buffers
+    def initGeom(self, xy):
+        """ugks_block.py:209-255; cellGeom runs on the device (:748-751)."""
+        Ni, Nj = self.Ni, self.Nj
+        self.xy_H = np.ascontiguousarray(xy, dtype=np.float64)
+        self.xy_D = L.DeviceArray.from_host(self.xy_H)
+        self.centre_D = L.DeviceArray((Ni, Nj, 2), fill=-1.0)
+        self.side_D = L.DeviceArray((Ni, Nj, 2, 2), fill=-1.0)
+        self.area_D = L.DeviceArray((Ni, Nj), fill=-1.0)
+        self.normal_D = L.DeviceArray((Ni, Nj, 2, 2), fill=-1.0)
+        self.length_D = L.DeviceArray((Ni, Nj, 2), fill=-1.0)
+        self.prg.cellGeom(self.stream, self.xy_D, self.area_D, self.centre_D, self.side_D,
+                          self.normal_D, self.length_D)
+        self.area_H = self.area_D.download(self.stream)
+        length_H = self.length_D.download(self.stream)
+        G = self.ghost
+        self.face_lengths = [length_H[G:-G, -G, 0], length_H[-G, G:-G, 1],
+                             length_H[G:-G, G, 0], length_H[G, G:-G, 1]]
+        self.total_area = float(np.sum(self.area_H[G:-G, G:-G]))
+        self.para_D = L.DeviceArray((max(self.ni, self.nj), 4, 4), fill=0.0)
+
+    def initFunctions(self):
+        """ugks_block.py:258-432 (fresh start; restart from HDF5 is not supported)."""
+        Ni, Nj, Nv, ni, nj = self.Ni, self.Nj, self.Nv, self.ni, self.nj
+        self.macro_H, self.Q_H = initial_macro(self.bdef)
+        self.flag_H = np.zeros(2)
+        self.flag_D = L.DeviceArray.from_host(self.flag_H)
+        dist = (Ni, Nj, Nv, 2)
+        self.f_D = L.DeviceArray(dist, fill=0.0)
+        self.flux_f_S_D = L.DeviceArray(dist, fill=0.0)
+        self.flux_f_W_D = L.DeviceArray(dist, fill=0.0)
+        self.sigma_D = L.DeviceArray(dist, fill=0.0)
+        self.flux_macro_S_D = L.DeviceArray((Ni, Nj, 4), fill=0.0)
+        self.flux_macro_W_D = L.DeviceArray((Ni, Nj, 4), fill=0.0)
+        self.prim_D = L.DeviceArray((Ni, Nj, 4), fill=0.0)
+        self.aL_D = L.DeviceArray((Ni, Nj, 4), fill=0.0)
+        self.aR_D = L.DeviceArray((Ni, Nj, 4), fill=0.0)
+        self.aT_D = L.DeviceArray((Ni, Nj, 4), fill=0.0)
+        self.faceQ_D = L.DeviceArray((Ni, Nj, 2), fill=0.0)
+        self.Mxi_D = L.DeviceArray((Ni, Nj, 3), fill=0.0)
+        wl = max(ni, nj)
+        self.wall_prop_D = L.DeviceArray((wl, 4, 4), fill=0.0)
+        cover = np.zeros((wl, 4, 4))
+        for bci, bcx in enumerate(self.bc_list):
+            cover[:, bci, 0] = bcx.cover_initial
+        self.wall_cover_D = L.DeviceArray.from_host(cover)
+        self.wall_fluxes_D = L.DeviceArray((wl, 4, 8), fill=0.0)
+        self.wall_dist_D = L.DeviceArray((wl, Nv, 2), fill=0.0)
+        self.macro_D = L.DeviceArray.from_host(self.macro_H)
+        self.Q_D = L.DeviceArray.from_host(self.Q_H)
+        self.time_step_D = L.DeviceArray((ni, nj), fill=0.0)
+        self.max_freq_D = L.DeviceArray((1,), fill=0.0)
+        self.residual_D = L.DeviceArray((ni, nj, 4), fill=0.0)
+        self.bc_mdot_D = L.DeviceArray((wl, 4), fill=0.0)
+        s = self.stream
+        self.prg.initFunctions(s, self.f_D, self.macro_D, self.Q_D)
+        self.prg.calcMacro(s, self.f_D, self.macro_D)
+        self.prg.calcQ(s, self.f_D, self.macro_D, self.Q_D)
+        self.parametricBC(force=True)
+
+    # ------------------------------------------------------------- BCs
+    def parametricBC(self, force=False):
+        """ugks_block.py:603-619 (constant wall values; no time-dependent UDFs)."""
+        if force:
+            self.prg.paraBC(self.stream, self.para_D, self.wall_prop_D, np.float64(self.gdata.time))
+
+    def ghostExchange(self, this_face, other_block, other_face, ori=1, flip=bcd.NO_FLIP):
+        """ugks_block.py:435-473"""
+        o = self.blockList[other_block]
+        self.prg.edgeExchange(self.stream, self.f_D, self.xy_D, np.int32(this_face), o.f_D, o.xy_D,
+                              np.int32(o.Ni), np.int32(o.Nj), np.int32(other_face), np.int32(ori), np.int32(flip))
+
+    def updateBC(self):
+        """ugks_block.py:621-665"""
+        s = self.stream
+        for this_face, b in enumerate(self.bc_list):
+            t = b.type_of_BC
+            f = np.int32(this_face)
+            if t in (bcd.ADJACENT, bcd.PERIODIC):
+                self.ghostExchange(this_face, b.other_block, b.other_face, b.orientation, b.flip_distribution)
+            elif t == bcd.EXTRAPOLATE_OUT:
+                self.prg.edgeExtrapolate(s, self.f_D, f)
+            elif t == bcd.CONSTANT:
+                self.prg.edgeConstant(s, self.f_D, f, self.wall_prop_D)
+            elif t == bcd.REFLECT:
+                self.prg.edgeMirror(s, self.f_D, f)
+            elif t == bcd.BOUNCE_BACK:
+                self.prg.edgeBounceBack(s, self.f_D, f)
+            elif t in (bcd.DIFFUSE, bcd.ADSORBING):
+                self.prg.edgeConstGrad(s, self.f_D, f)
+                self.has_accommodating = True
+            elif t == bcd.INFLOW:
+                self.prg.edgeInflow(s, self.f_D, f, self.normal_D, self.macro_D, self.Q_D, self.wall_prop_D)
+            elif t == bcd.OUTFLOW:
+                self.prg.edgeOutflow(s, self.f_D, f, self.normal_D, self.macro_D, self.Q_D, self.wall_prop_D)
+
+    # ------------------------------------------------------------- flags
+    def get_flag(self):
+        """ugks_block.py:840-876 (blocking read of the device flag)."""
+        self.flag_H = self.flag_D.download(self.stream)
+        code = self.flag_H[0]
+        if code in (1.0, 1.1, 1.2, 1.3, 4.0):
+            self.last_error = {1.0: "NaN encountered", 4.0: "pressure / temperature outside adsorption isotherm data"}.get(code, "NaN")
+            return True
+        if code in (2.0, 3.0):
+            self.max_dt = min(self.max_dt, self.flag_H[1])
+        return False
+
+    def reset_flag(self):
+        """ugks_block.py:878-887"""
+        self.flag_D.zero(self.stream)
+
+    # ------------------------------------------------------------- UGKS
+    def wall_flux(self, bcx, wall, flux, flux_macro, dt):
+        """ugks_block.py:930-983"""
+        s = self.stream
+        w = np.int32(wall)
+        if bcx.type_of_BC == bcd.DIFFUSE:
+            self.prg.accommodatingWallDist(s, self.normal_D, w, self.wall_prop_D, flux, dt)
+        else:
+            self.reset_flag()
+            self.prg.adsorbingWall_P1(s, self.normal_D, w, self.wall_prop_D, self.wall_cover_D, self.wall_dist_D,
+                                      flux, self.macro_D, dt, self.flag_D)
+            if self.get_flag():
+                return True
+            if bcx.reflect_type == "CL":
+                raise NotImplementedError("Cercignani-Lampis re-emission (adsorbingWallCL_P2) is not implemented")
+            elif bcx.reflect_type == "D":
+                self.prg.adsorbingWallD_P2(s, self.normal_D, w, self.wall_prop_D, self.wall_cover_D,
+                                           self.wall_dist_D, flux, self.macro_D, dt)
+            else:
+                self.prg.adsorbingWallS_P2(s, self.normal_D, w, self.wall_prop_D, self.wall_cover_D,
+                                           self.wall_dist_D, flux, self.macro_D, dt)
+        self.prg.wallFlux(s, self.normal_D, self.length_D, w, flux, flux_macro, dt)
+        self.prg.wallMassEnergyFluxes(s, self.normal_D, self.length_D, self.area_D, w, flux, self.wall_fluxes_D, dt)
+        return False
+
+    def _flux_pass(self, face, dt):
+        """One direction of UGKS_flux (ugks_block.py:1062-1156 / :1167-1257)."""
+        s, p = self.stream, self.prg
+        ff = self.flux_f_S_D if face == SOUTH else self.flux_f_W_D
+        fm = self.flux_macro_S_D if face == SOUTH else self.flux_macro_W_D
+        top, bot = (bcd.NORTH, bcd.SOUTH) if face == SOUTH else (bcd.EAST, bcd.WEST)
+        p.iFace(s, self.f_D, ff, self.sigma_D, self.centre_D, self.side_D, self.normal_D, face)
+        ot = ob = 0
+        if self.bc_list[top].type_of_BC in (bcd.DIFFUSE, bcd.ADSORBING):
+            ot = 1
+            self.wall_flux(self.bc_list[top], top, ff, fm, dt)
+        if self.bc_list[bot].type_of_BC in (bcd.DIFFUSE, bcd.ADSORBING):
+            ob = 1
+            self.wall_flux(self.bc_list[bot], bot, ff, fm, dt)
+        ob, ot = np.int32(ob), np.int32(ot)
+        p.getFaceCons(s, ff, self.normal_D, face, self.prim_D, ob, ot)
+        p.getAL(s, self.f_D, self.normal_D, face, self.aL_D, ob, ot)
+        p.getAR(s, self.f_D, self.normal_D, face, self.aR_D, ob, ot)
+        p.getPLR(s, self.centre_D, self.side_D, face, self.prim_D, self.aL_D, self.aR_D, ob, ot)
+        p.initMacroFlux(s, fm, face, dt, self.prim_D, self.aL_D, self.aR_D, self.aT_D, self.Mxi_D, ob, ot)
+        p.calcFaceQ(s, ff, self.prim_D, self.normal_D, self.faceQ_D, face, ob, ot)
+        p.macroFlux(s, ff, self.sigma_D, fm, self.normal_D, self.length_D, face, dt, self.prim_D, self.faceQ_D, ob, ot)
+        p.distFlux(s, ff, self.sigma_D, self.normal_D, self.length_D, face, dt, self.prim_D, self.aL_D,
+                   self.aR_D, self.aT_D, self.Mxi_D, self.faceQ_D, ob, ot)
+
+    def UGKS_flux(self):
+        """ugks_block.py:1049-1260"""
+        dt = np.float64(self.gdata.dt)
+        self._flux_pass(SOUTH, dt)
+        self._flux_pass(WEST, dt)
+        return False
+
+    def UGKS_update(self, get_residual=False):
+        """ugks_block.py:1262-1329"""
+        s, p = self.stream, self.prg
+        p.calcQ(s, self.f_D, self.macro_D, self.Q_D)
+        p.updateMacro(s, self.flux_macro_S_D, self.flux_macro_W_D, self.area_D, self.macro_D, self.residual_D)
+        self.reset_flag()
+        dt = np.float64(self.gdata.dt)
+        p.UGKS_update(s, self.f_D, self.flux_f_S_D, self.flux_f_W_D, self.area_D, self.macro_D, self.Q_D,
+                      self.residual_D, dt, self.flag_D)
+        if self.get_flag():
+            return True
+        if get_residual:
+            self.get_BC_mdot()
+            p.getResidual(s, self.macro_D, self.residual_D)
+            res = self.residual_D.download(s)
+            macro = self.macro_D.download(s)
+            sum_res = np.sum(np.sum(res, axis=0), axis=0)
+            sum_avg = np.sum(np.sum(np.abs(macro), axis=0), axis=0)
+            self.residual = np.sqrt(self.ni * self.nj * sum_res) / (sum_avg + np.finfo(float).tiny)
+            self.residual[self.residual <= 0.0] = 1.0
+        return False
+
+    def get_BC_mdot(self):
+        """ugks_block.py:889-924"""
+        got = False
+        for this_face, b in enumerate(self.bc_list):
+            if b.type_of_BC in (bcd.EXTRAPOLATE_OUT, bcd.INFLOW, bcd.OUTFLOW, bcd.CONSTANT):
+                self.prg.edgeMassFlow(self.stream, self.macro_D, self.normal_D, self.length_D, self.bc_mdot_D,
+                                      np.int32(this_face))
+                got = True
+        self.bc_mdot = float(np.sum(self.bc_mdot_D.download(self.stream))) if got else 0.0
+
+    def getDT(self):
+        """ugks_block.py:1415-1433"""
+        g = self.gdata
+        self.prg.clFindDT(self.stream, self.xy_D, self.area_D, self.macro_D, self.time_step_D, np.float64(g.Kn_eff))
+        self.prg.time_step_max(self.stream, self.time_step_D, self.max_freq_D)
+        max_freq = float(self.max_freq_D.download(self.stream)[0])
+        self.max_dt = g.CFL / max_freq
+        if g.suggest_dt:
+            self.max_dt = min(g.suggest_dt, self.max_dt)
+
+    # ------------------------------------------------------------- host data
+    def macro(self):
+        return self.macro_D.download(self.stream)
+
+    def f(self):
+        return self.f_D.download(self.stream)
+
+    def Q(self, external=False):
+        """calcQ on the current state (ugks_block.py:1377-1386)."""
+        self.prg.calcQ(self.stream, self.f_D, self.macro_D, self.Q_D)
+        q = self.Q_D.download(self.stream)
+        if external:
+            q = q * (4 + self.gdata.K) / 5.0
+        return q
+
+    def block_mass(self):
+        """ugks_block.py:1598-1623"""
+        G = self.ghost
+        macro = self.macro()
+        bulk = float(np.sum(self.area_H[G:-G, G:-G] * macro[:, :, 0])) * self.gdata.D_ref * self.gdata.L_ref ** 2
+        cover = self.wall_cover_D.download(self.stream)
+        ads = 0.0
+        lens = [self.ni, self.nj, self.ni, self.nj]
+        for face, b in enumerate(self.bc_list):
+            ads += float(np.sum(cover[0:lens[face], face, 0] * b.S_T * self.face_lengths[face] * self.gdata.L_ref))
+        return bulk, ads

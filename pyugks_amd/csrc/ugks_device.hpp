@@ -1,0 +1,258 @@
+// ugks_device.hpp -- shared device types and the per-velocity physics of the
+// UGKS step (gfx950).  The formulas follow clFunctions.c / clUGKS.c of the
+// reference (cited per function); the code is written for wave64 kernels
+// and keeps every operand in FP64.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ugks {
+
+constexpr double kPi = 3.141592653589793;   // '%0.15e' % pi (ugks_CL.py:84)
+enum { SOUTH = 0, WEST = 1 };
+enum { GNORTH = 0, GEAST = 1, GSOUTH = 2, GWEST = 3 };
+enum { NO_FLIP = 0, FLIP_NS = 1, FLIP_D = 2, FLIP_EW = 3 };
+constexpr double ERR_NAN = 1.0;
+constexpr double ERR_CLAMPING_ADSORB_ISOTHERM = 4.0;
+
+struct __align__(16) d2 { double x, y; };
+struct __align__(16) d4 { double s0, s1, s2, s3; };
+
+__host__ __device__ inline d2 mk2(double x, double y) { return d2{x, y}; }
+__host__ __device__ inline d4 mk4(double a, double b, double c, double d) { return d4{a, b, c, d}; }
+__host__ __device__ inline d2 operator+(d2 a, d2 b) { return d2{a.x + b.x, a.y + b.y}; }
+__host__ __device__ inline d2 operator-(d2 a, d2 b) { return d2{a.x - b.x, a.y - b.y}; }
+__host__ __device__ inline d2 operator*(double s, d2 a) { return d2{s * a.x, s * a.y}; }
+__host__ __device__ inline d2 operator*(d2 a, double s) { return d2{a.x * s, a.y * s}; }
+__host__ __device__ inline d2 operator/(d2 a, double s) { return d2{a.x / s, a.y / s}; }
+__host__ __device__ inline d4 operator+(d4 a, d4 b) { return d4{a.s0 + b.s0, a.s1 + b.s1, a.s2 + b.s2, a.s3 + b.s3}; }
+__host__ __device__ inline d4 operator-(d4 a, d4 b) { return d4{a.s0 - b.s0, a.s1 - b.s1, a.s2 - b.s2, a.s3 - b.s3}; }
+__host__ __device__ inline d4 operator*(double s, d4 a) { return d4{s * a.s0, s * a.s1, s * a.s2, s * a.s3}; }
+__host__ __device__ inline d4 operator/(d4 a, double s) { return d4{a.s0 / s, a.s1 / s, a.s2 / s, a.s3 / s}; }
+__host__ __device__ inline double dot(d2 a, d2 b) { return a.x * b.x + a.y * b.y; }
+__host__ __device__ inline double length(d2 a) { return sqrt(a.x * a.x + a.y * a.y); }
+
+// Per-block constants passed by value to every kernel: the runtime form of
+// the genHeader #defines (ugks_CL.py:58-395).
+struct Consts {
+    int ni, nj, NI, NJ, NV, G, method, relax;
+    double Pr, Kn, gam, K, omega, umax, vmax;
+    double wall_D[4], wall_U[4], wall_V[4], wall_T[4], wall_P[4];
+    double beta_n[4], beta_t[4], gamma_f[4], alpha_p[4], adsorb[4];
+    int dyn[4], cads[4];
+    const d2 *quad;
+    const double *weight;
+    const int64_t *mNS, *mEW, *mD;
+};
+
+// ---- indexing of the reference buffers (clKernel.c:4-26) ------------------
+__device__ __forceinline__ size_t fidx(const Consts &C, int i, int j, int v) {
+    return ((size_t)i * C.NJ + (size_t)j) * (size_t)C.NV + (size_t)v;
+}
+__device__ __forceinline__ size_t cidx(const Consts &C, int i, int j) { return (size_t)i * C.NJ + j; }
+__device__ __forceinline__ size_t midx(const Consts &C, int i, int j) { return (size_t)i * C.nj + j; }
+__device__ __forceinline__ size_t sidx(const Consts &C, int i, int j, int face) {
+    return ((size_t)i * C.NJ + (size_t)j) * 2 + face;
+}
+
+// OpenCL sign(): +-1, the signed zero itself, 0 for NaN
+__device__ __forceinline__ double ocl_sign(double x) {
+    return x > 0.0 ? 1.0 : (x < 0.0 ? -1.0 : (x == 0.0 ? x : 0.0));
+}
+// int delta = (sign(u)+1)/2 : 1 for u > 0, else 0 (clUGKS.c:148, :743)
+__device__ __forceinline__ int upwind(double u) { return u > 0.0 ? 1 : 0; }
+// int delta = (sign(u)*rot+1)/2 : 1 when the velocity leaves the wall (clUGKS.c:852)
+__device__ __forceinline__ int leaves_wall(double u, int rot) { return (rot > 0) ? (u > 0.0) : (u < 0.0); }
+
+// interfaceVelocity (clFunctions.c:191-209): (normal, tangential) components
+__device__ __forceinline__ d2 face_velocity(const Consts &C, int v, d2 n) {
+    d2 e = C.quad[v];
+    return d2{e.x * n.x + e.y * n.y, e.x * (-n.y) + e.y * n.x};
+}
+// toGlobal (clFunctions.c:215-230) and toLocal (:236-250)
+__device__ __forceinline__ d2 to_global(d2 in, d2 n) {
+    return d2{n.x * in.x + (-n.y) * in.y, n.y * in.x + n.x * in.y};
+}
+__device__ __forceinline__ d2 to_local(d2 in, d2 n) {
+    return d2{in.x * n.x + in.y * n.y, in.x * (-n.y) + in.y * n.x};
+}
+
+// relaxTime (clFunctions.c:8-32)
+__device__ __forceinline__ double relax_time(const Consts &C, d4 p) {
+    double base = (C.Kn / p.s0);
+    double pw = pow(p.s3, 1.0 - C.omega);
+    switch (C.relax) {
+    case 0: return base * 0.79788456080286541 * pw;
+    case 1: return (5. / 8.) * base * sqrt(kPi) * pw;
+    case 3: return base * 2.2567583341910251 * pw;
+    default: return base * pw;
+    }
+}
+
+// fM (clFunctions.c:49-58): weighted Maxwellian (g, h)
+__device__ __forceinline__ d2 maxwell(const Consts &C, d4 p, d2 uv, int v) {
+    double cx = uv.x - p.s1, cy = uv.y - p.s2;
+    double g = p.s0 * (p.s3 / kPi) * exp(-p.s3 * (cx * cx + cy * cy));
+    double h = (g * C.K) / (2.0 * p.s3);
+    double w = C.weight[v];
+    return d2{w * g, w * h};
+}
+// fS (clFunctions.c:64-77): Shakhov correction of a Maxwellian M
+__device__ __forceinline__ d2 shakhov(const Consts &C, d4 p, d2 Q, d2 uv, d2 M) {
+    double cx = uv.x - p.s1, cy = uv.y - p.s2;
+    double s = 0.8 * (1 - C.Pr) * (p.s3 * p.s3) / p.s0 * (cx * Q.x + cy * Q.y);
+    double part = 2 * p.s3 * (cx * cx + cy * cy);
+    return d2{s * (part - 4) * M.x, s * (part - 2) * M.y};
+}
+// fEQ (clFunctions.c:79-86)
+__device__ __forceinline__ d2 shakhov_eq(const Consts &C, d4 p, d2 Q, d2 uv, int v) {
+    d2 M = maxwell(C, p, uv, v);
+    return M + shakhov(C, p, Q, uv, M);
+}
+
+// vanLeer (clFunctions.c:256-260), one component
+__device__ __forceinline__ double van_leer(double s1, double s2) {
+    return (ocl_sign(s1) + ocl_sign(s2)) * (fabs(s1) * fabs(s2)) / (fabs(s1) + fabs(s2) + 1e-16);
+}
+// WENO5 slope (clFunctions.c:268-312), one component; s0..s4 upwind-ordered
+__device__ __forceinline__ double weno5(double s0, double s1, double s2, double s3, double s4, double dx) {
+    double t1 = s0 - 2.0 * s1 + s2, t2 = s0 - 4.0 * s1 + 3.0 * s2;
+    double b0 = (13.0 / 12.0) * t1 * t1 + 0.25 * t2 * t2;
+    t1 = s1 - 2.0 * s2 + s3; t2 = s1 - s3;
+    double b1 = (13.0 / 12.0) * t1 * t1 + 0.25 * t2 * t2;
+    t1 = s2 - 2.0 * s3 + s4; t2 = 3.0 * s2 - 4.0 * s3 + s4;
+    double b2 = (13.0 / 12.0) * t1 * t1 + 0.25 * t2 * t2;
+    double e0 = 1E-16 + b0, e1 = 1E-16 + b1, e2 = 1E-16 + b2;
+    double a0 = 0.1 / (e0 * e0), a1 = 0.6 / (e1 * e1), a2 = 0.3 / (e2 * e2);
+    double sum = a0 + a1 + a2;
+    double w0 = a0 / sum, w1 = a1 / sum, w2 = a2 / sum;
+    double f0 = (1. / 3.) * s0 - (7. / 6.) * s1 + (11. / 6.) * s2;
+    double f1 = -(1. / 6.) * s1 + (5. / 6.) * s2 + (1. / 3.) * s3;
+    double f2 = (1. / 3.) * s2 + (5. / 6.) * s3 - (1. / 6.) * s4;
+    return ((w0 * f0 + w1 * f1 + w2 * f2) - s2) / dx;
+}
+// minmod + NND (clFunctions.c:320-352), one component
+__device__ __forceinline__ double minmod(double a, double b) {
+    if ((fabs(a) < fabs(b)) && (a * b > 0)) return a;
+    if ((fabs(b) < fabs(a)) && (a * b > 0)) return b;
+    if (a == b) return a;
+    return 0.0;
+}
+__device__ __forceinline__ double nnd(double s0, double s1, double s2, double dx) {
+    double ip = s1 + 0.5 * minmod(s2 - s1, s1 - s0);
+    return (ip - s1) / dx;
+}
+
+// getPrimary / getConserved (clUGKS.c:16-28)
+__device__ __forceinline__ d4 to_primary(const Consts &C, d4 w) {
+    d4 p;
+    p.s3 = 0.5 * w.s0 / (C.gam - 1.0) / (w.s3 - 0.5 * (w.s1 * w.s1 + w.s2 * w.s2) / w.s0);
+    p.s0 = w.s0;
+    p.s1 = w.s1 / w.s0;
+    p.s2 = w.s2 / w.s0;
+    return p;
+}
+__device__ __forceinline__ d4 to_conserved(const Consts &C, d4 p) {
+    d4 w;
+    w.s3 = 0.5 * p.s0 / p.s3 / (C.gam - 1.0) + 0.5 * p.s0 * (p.s1 * p.s1 + p.s2 * p.s2);
+    w.s0 = p.s0;
+    w.s1 = p.s1 * p.s0;
+    w.s2 = p.s2 * p.s0;
+    return w;
+}
+
+// microSlope (clUGKS.c:30-45): Maxwellian slope coefficients from dW
+__device__ __forceinline__ d4 micro_slope(const Consts &C, d4 p, d4 sw) {
+    const double K = C.K;
+    d4 m;
+    m.s3 = 4.0 * (p.s3 * p.s3) / (K + 2) / p.s0 *
+           (2.0 * sw.s3 - 2.0 * p.s1 * sw.s1 - 2.0 * p.s2 * sw.s2 +
+            sw.s0 * ((p.s1 * p.s1 + p.s2 * p.s2) - 0.5 * (K + 2) / p.s3));
+    m.s2 = 2.0 * p.s3 / p.s0 * (sw.s2 - p.s2 * sw.s0) - p.s2 * m.s3;
+    m.s1 = 2.0 * p.s3 / p.s0 * (sw.s1 - p.s1 * sw.s0) - p.s1 * m.s3;
+    m.s0 = sw.s0 / p.s0 - p.s1 * m.s1 - p.s2 * m.s2 -
+           0.5 * ((p.s1 * p.s1) + (p.s2 * p.s2) + 0.5 * (K + 2) / p.s3) * m.s3;
+    return m;
+}
+
+// Half/full-range Maxwellian moments (momentU, clUGKS.c:47-80)
+struct Moments {
+    double Mu[7], MuL[7], MuR[7], Mv[5], Mxi[3];
+};
+__device__ inline void velocity_moments(const Consts &C, d4 p, Moments &m) {
+    double sl = sqrt(p.s3);
+    double ex = 0.5 * exp(-p.s3 * (p.s1 * p.s1)) / sqrt(kPi * p.s3);
+    m.MuL[0] = 0.5 * erfc(-sl * p.s1);
+    m.MuL[1] = p.s1 * m.MuL[0] + ex;
+    m.MuR[0] = 0.5 * erfc(sl * p.s1);
+    m.MuR[1] = p.s1 * m.MuR[0] - ex;
+#pragma unroll
+    for (int i = 2; i < 7; ++i) {
+        m.MuL[i] = p.s1 * m.MuL[i - 1] + 0.5 * (i - 1) * m.MuL[i - 2] / p.s3;
+        m.MuR[i] = p.s1 * m.MuR[i - 1] + 0.5 * (i - 1) * m.MuR[i - 2] / p.s3;
+    }
+#pragma unroll
+    for (int i = 0; i < 7; ++i) m.Mu[i] = m.MuL[i] + m.MuR[i];
+    m.Mv[0] = 1.0;
+    m.Mv[1] = p.s2;
+#pragma unroll
+    for (int i = 2; i < 5; ++i) m.Mv[i] = p.s2 * m.Mv[i - 1] + 0.5 * (i - 1) * m.Mv[i - 2] / p.s3;
+    m.Mxi[0] = 1.0;
+    m.Mxi[1] = 0.5 * C.K / p.s3;
+    m.Mxi[2] = (C.K * C.K + 2.0 * C.K) / (4.0 * p.s3 * p.s3);
+}
+// moment_uv (clUGKS.c:82-93): <u^a v^b xi^d psi>
+__device__ __forceinline__ d4 moment_uv(const double *Mu, const double *Mv, const double *Mxi, int a, int b, int d) {
+    d4 r;
+    r.s0 = Mu[a] * Mv[b] * Mxi[d / 2];
+    r.s1 = Mu[a + 1] * Mv[b] * Mxi[d / 2];
+    r.s2 = Mu[a] * Mv[b + 1] * Mxi[d / 2];
+    r.s3 = 0.5 * (Mu[a + 2] * Mv[b] * Mxi[d / 2] + Mu[a] * Mv[b + 2] * Mxi[d / 2] + Mu[a] * Mv[b] * Mxi[(d + 2) / 2]);
+    return r;
+}
+// moment_au (clUGKS.c:95-104): <a u^a v^b psi>
+__device__ __forceinline__ d4 moment_au(d4 A, const double *Mu, const double *Mv, const double *Mxi, int a, int b) {
+    return A.s0 * moment_uv(Mu, Mv, Mxi, a, b, 0) + A.s1 * moment_uv(Mu, Mv, Mxi, a + 1, b, 0) +
+           A.s2 * moment_uv(Mu, Mv, Mxi, a, b + 1, 0) + (0.5 * A.s3) * moment_uv(Mu, Mv, Mxi, a + 2, b, 0) +
+           (0.5 * A.s3) * moment_uv(Mu, Mv, Mxi, a, b + 2, 0) + (0.5 * A.s3) * moment_uv(Mu, Mv, Mxi, a, b, 2);
+}
+
+// Time integrals Mt[0..4] of the UGKS flux (clUGKS.c:499-503)
+struct TimeInt { double m0, m1, m2, m3, m4; };
+__device__ __forceinline__ TimeInt time_integrals(double tau, double dt) {
+    TimeInt t;
+    double e = exp(-dt / tau);
+    t.m3 = tau * (1.0 - e);
+    t.m4 = -tau * dt * e + tau * t.m3;
+    t.m0 = dt - t.m3;
+    t.m1 = -tau * t.m0 + t.m4;
+    t.m2 = (dt * dt) / 2.0 - tau * t.m0;
+    return t;
+}
+
+// distFlux per-velocity flux (clUGKS.c:741-763), before the length factor
+__device__ __forceinline__ d2 ugks_face_flux(const Consts &C, d4 prim, d4 aL, d4 aR, d4 aT, double Mxi2,
+                                             const TimeInt &mt, d2 Q, d2 uv, int v, d2 fface, d2 fslope) {
+    d2 f0 = maxwell(C, prim, uv, v);
+    d2 F0 = shakhov(C, prim, Q, uv, f0);
+    double uu = uv.x * uv.x + uv.y * uv.y, ux2 = uv.x * uv.x;
+    d4 a = upwind(uv.x) ? aL : aR;
+    double pg = aT.s0 * f0.x + aT.s1 * uv.x * f0.x + aT.s2 * uv.y * f0.x + 0.5 * aT.s3 * (uu * f0.x + f0.y);
+    double ph = aT.s0 * f0.y + aT.s1 * uv.x * f0.y + aT.s2 * uv.y * f0.y + 0.5 * aT.s3 * (uu * f0.y + Mxi2 * f0.x);
+    double sg = a.s0 * f0.x + a.s1 * uv.x * f0.x + a.s2 * uv.y * f0.x + 0.5 * a.s3 * (uu * f0.x + f0.y);
+    double sh = a.s0 * f0.y + a.s1 * uv.x * f0.y + a.s2 * uv.y * f0.y + 0.5 * a.s3 * (uu * f0.y + Mxi2 * f0.x);
+    d2 r;
+    r.x = mt.m0 * uv.x * (f0.x + F0.x) + mt.m1 * ux2 * sg + mt.m2 * uv.x * pg + mt.m3 * uv.x * fface.x - mt.m4 * ux2 * fslope.x;
+    r.y = mt.m0 * uv.x * (f0.y + F0.y) + mt.m1 * ux2 * sh + mt.m2 * uv.x * ph + mt.m3 * uv.x * fface.y - mt.m4 * ux2 * fslope.y;
+    return r;
+}
+
+// ---- wave64 helpers ---------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double x) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    return x;
+}
+
+}  // namespace ugks
