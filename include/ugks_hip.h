@@ -181,6 +181,58 @@ int ugks_edgeBounceBack(ugks_block *blk, void *stream, void *Fin, int32_t this_f
 /* clGhostRoutines.c:967-968, ugks_block.py:600-601 */
 int ugks_edgeConstGrad(ugks_block *blk, void *stream, void *Fin, int32_t this_face);
 
+/* ---- fused, batched step (layer 2) -------------------------------------- */
+/* One plan steps any number of blocks (of one or many independent cases) in
+ * six launches per step with the time step kept on the device.  It replaces
+ * the sequence UGKSim.one_step drives per block (ugks_sim.py:407-501:
+ * UGKS_flux, UGKS_update, calcTimeStep/updateTimeStep, updateAllBC) and
+ * produces the same f, macro, Q, flux_macro, wall_cover and wall_fluxes.
+ * Blocks given to one plan share one time step (ugks_sim.py:296-308);
+ * independent cases that need their own time steps get their own plans. */
+enum { UGKS_FP64 = 0, UGKS_MIXED = 1 };
+
+/* Device buffers of one block, in the layouts of layer 1.  f, flux_f_S,
+ * flux_f_W and wall_dist hold (g,h) pairs of double for UGKS_FP64 and of
+ * float for UGKS_MIXED; everything else is double.  macro_new is a second
+ * [ni][nj] double4 buffer (the step writes the new state there and copies it
+ * back to macro). */
+typedef struct ugks_block_buffers {
+    void *f, *flux_f_S, *flux_f_W, *flux_macro_S, *flux_macro_W;
+    void *macro, *macro_new, *residual, *Q;
+    const void *xy, *centre, *mid_side, *normal, *area, *side_length;
+    void *wall_prop, *wall_cover, *wall_fluxes, *wall_dist, *time_step, *flag;
+} ugks_block_buffers;
+
+/* One face's BC (geom/geom_bc_defs.py:27-51): type code, partner block and
+ * face for ADJACENT, orientation flag, distribution flip, reflect type of an
+ * adsorbing wall (0 specular, 1 diffuse). */
+typedef struct ugks_bc_desc {
+    int32_t type, other_block, other_face, orientation, flip, reflect_type;
+} ugks_bc_desc;
+
+/* Time-step state (the gdata fields of ugks_sim.py:285-311, :491). */
+typedef struct ugks_dt_desc {
+    double dt, time, CFL, delta_dt, suggest_dt, Kn_eff;
+    int32_t step, overshoot;
+} ugks_dt_desc;
+
+typedef struct ugks_step_plan ugks_step_plan;
+
+int ugks_step_plan_create(ugks_block *const *blocks, const ugks_block_buffers *bufs, const ugks_bc_desc *bcs,
+                          int32_t nblocks, int32_t precision, const ugks_dt_desc *dt, ugks_step_plan **out);
+int ugks_step_plan_destroy(ugks_step_plan *plan);
+/* nsteps full steps: flux, update, time step, BC refresh (ugks_sim.py:407-501) */
+int ugks_step_run(ugks_step_plan *plan, void *stream, int32_t nsteps);
+/* the same steps with HIP events around each launch; kernel_ms[6] receives the
+ * summed time of face, wall, update, dt, dt-finalise and ghost launches */
+int ugks_step_run_timed(ugks_step_plan *plan, void *stream, int32_t nsteps, float *kernel_ms);
+/* read back dt, time, step (and the per-block maximum time steps if max_dt != NULL) */
+int ugks_step_state(ugks_step_plan *plan, void *stream, ugks_dt_desc *out, double *max_dt);
+int ugks_step_set_dt(ugks_step_plan *plan, void *stream, double dt);
+/* element-wise precision conversion of n values (f of the mixed-precision step) */
+int ugks_f64_to_f32(void *stream, const void *src, void *dst, int64_t n);
+int ugks_f32_to_f64(void *stream, const void *src, void *dst, int64_t n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -44,6 +44,10 @@ struct Consts {
     const d2 *quad;
     const double *weight;
     const int64_t *mNS, *mEW, *mD;
+    // tensor structure of the velocity set: quad[v] = (qu[v / nvt], qv[v % nvt]),
+    // so a Maxwellian factorises into per-axis exponentials (nut = 0 if not a tensor grid)
+    int nut, nvt;
+    const double *qu, *qv;
 };
 
 // ---- indexing of the reference buffers (clKernel.c:4-26) ------------------

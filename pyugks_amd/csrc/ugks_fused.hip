@@ -1,0 +1,1183 @@
+// ugks_fused.hip -- the batched, fused UGKS time step for gfx950.
+//
+// One step of every block of every case in six launches, with no host
+// round trip (the reference issues ~30 launches per block per step plus two
+// blocking readbacks, ugks_block.py:1049-1329, :1415-1433):
+//
+//   k_face    one workgroup per interior face (S and W faces of all blocks):
+//             reconstruction (iFace), face/left/right moments
+//             (getFaceCons/getAL/getAR), slopes and time slope
+//             (getPLR/initMacroFlux), face heat flux (calcFaceQ), macro flux
+//             (macroFlux) and the per-velocity flux (distFlux); the owning
+//             cell's heat flux (calcQ) rides along on S faces.
+//   k_wall    one workgroup per diffuse/adsorbing wall face: reconstruction,
+//             accommodatingWallDist or adsorbingWall_P1 + S/D_P2, wallFlux,
+//             wallMassEnergyFluxes.
+//   k_update  one workgroup per cell: updateMacro + UGKS_update.
+//   k_dt      one thread per cell: clFindDT and the per-block maximum.
+//   k_dtfin   one workgroup: the global time step with the reference's growth
+//             limit (ugks_sim.py:285-311, applied as one_step does, :430-446).
+//   k_ghost   one workgroup per ghost cell: every BC of every block
+//             (updateBC, ugks_block.py:621-665).
+//
+// Items of all blocks are laid out back to back; a workgroup finds its block
+// from a prefix table.  f is stored as T2 = (g, h) pairs of T (double, or
+// float for the mixed-precision step); moment sums, the macroscopic update,
+// heat fluxes, equilibrium moments and the time step are FP64 in both.
+// Thread t of a workgroup owns velocities v = t + 256 k: every wavefront
+// load or store touches 64 consecutive velocities of one cell.
+#include <stdio.h>
+#include <string.h>
+
+#include <vector>
+
+#include "../../include/ugks_hip.h"
+#include "ugks_device.hpp"
+#include "ugks_runtime.hpp"
+
+using namespace ugks;
+
+namespace {
+
+constexpr int kT = 256;           // threads per workgroup
+constexpr int kWavesT = kT / 64;
+
+template <typename T> struct __align__(2 * sizeof(T)) P2 { T x, y; };
+
+// device record of one block for the fused kernels
+struct FBlock {
+    Consts c;
+    void *f, *fluxS, *fluxW, *wdist;       // P2<T> arrays ([NI][NJ][NV]; wdist [4][wl][NV])
+    d4 *fmS, *fmW, *macro, *macro_new, *residual, *wall_cover;
+    d2 *Q;
+    const d2 *xy, *centre, *mid_side, *normal;
+    const double *area, *side_length;
+    const d4 *wall_prop;
+    double *wall_fluxes, *time_step, *flag;
+    int bc[4], other[4], oface[4], ori[4], flip[4], reflect[4];
+    int nS, nSj, ob_S, ob_W, nWi, wl;       // interior-face counts and offsets
+    int flipidx[4];                         // exchange row flip from the corner check
+};
+
+struct DtState {
+    double dt, time, CFL, delta_dt, suggest_dt, Kn;
+    int step, overshoot;
+};
+
+__device__ __forceinline__ int find_block(const long long *off, int nb, long long item) {
+    int b = 0;
+    while (b + 1 < nb && off[b + 1] <= item) b++;
+    return b;
+}
+
+template <typename T> __device__ __forceinline__ P2<T> ld(const void *p, size_t k) { return ((const P2<T> *)p)[k]; }
+template <typename T> __device__ __forceinline__ void st(void *p, size_t k, T x, T y) { ((P2<T> *)p)[k] = P2<T>{x, y}; }
+
+// ---- block-wide sum of N doubles (every thread receives the totals) --------
+template <int N> __device__ __forceinline__ void block_sum(double (&a)[N], double (*red)[16]) {
+#pragma unroll
+    for (int q = 0; q < N; q++) a[q] = wave_sum(a[q]);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int q = 0; q < N; q++) red[w][q] = a[q];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < N; q++) a[q] = (red[0][q] + red[1][q]) + (red[2][q] + red[3][q]);
+    __syncthreads();
+}
+
+// ---- reconstruction (iFace, clUGKS.c:134-195) in precision T ---------------
+template <typename T> __device__ __forceinline__ T weno5_T(T s0, T s1, T s2, T s3, T s4, T rdx) {
+    T t1 = s0 - T(2) * s1 + s2, t2 = s0 - T(4) * s1 + T(3) * s2;
+    T b0 = T(13.0 / 12.0) * t1 * t1 + T(0.25) * t2 * t2;
+    t1 = s1 - T(2) * s2 + s3; t2 = s1 - s3;
+    T b1 = T(13.0 / 12.0) * t1 * t1 + T(0.25) * t2 * t2;
+    t1 = s2 - T(2) * s3 + s4; t2 = T(3) * s2 - T(4) * s3 + s4;
+    T b2 = T(13.0 / 12.0) * t1 * t1 + T(0.25) * t2 * t2;
+    T e0 = T(1e-16) + b0, e1 = T(1e-16) + b1, e2 = T(1e-16) + b2;
+    T a0 = T(0.1) / (e0 * e0), a1 = T(0.6) / (e1 * e1), a2 = T(0.3) / (e2 * e2);
+    T f0 = T(1. / 3.) * s0 - T(7. / 6.) * s1 + T(11. / 6.) * s2;
+    T f1 = -T(1. / 6.) * s1 + T(5. / 6.) * s2 + T(1. / 3.) * s3;
+    T f2 = T(1. / 3.) * s2 + T(5. / 6.) * s3 - T(1. / 6.) * s4;
+    T fL = (a0 * f0 + a1 * f1 + a2 * f2) / (a0 + a1 + a2);
+    return (fL - s2) * rdx;
+}
+template <typename T> __device__ __forceinline__ T vanleer_T(T s1, T s2) {
+    T a1 = s1 < T(0) ? -s1 : s1, a2 = s2 < T(0) ? -s2 : s2;
+    T sg = (s1 > T(0) ? T(1) : (s1 < T(0) ? T(-1) : T(0))) + (s2 > T(0) ? T(1) : (s2 < T(0) ? T(-1) : T(0)));
+    return sg * (a1 * a2) / (a1 + a2 + T(1e-16));
+}
+template <typename T> __device__ __forceinline__ T minmod_T(T a, T b) {
+    T fa = a < T(0) ? -a : a, fb = b < T(0) ? -b : b;
+    if ((fa < fb) && (a * b > T(0))) return a;
+    if ((fb < fa) && (a * b > T(0))) return b;
+    if (a == b) return a;
+    return T(0);
+}
+
+// per-face geometry of both upwind choices
+struct FaceGeo {
+    size_t own;       // owner cell (gi, gj) linear index
+    long long step;   // cells per unit step along the normal (1 for S, NJ for W)
+    double dist[2];   // |midside - c[MID]| for delta = 0, 1
+    double d1[2];     // vanLeer divisor |c[MID] - c[MID-1]| for delta = 0, 1
+};
+
+__device__ inline FaceGeo face_geo(const FBlock &B, int gi, int gj, int face) {
+    const Consts &C = B.c;
+    FaceGeo g;
+    g.own = (size_t)gi * C.NJ + gj;
+    g.step = face == SOUTH ? 1 : C.NJ;
+    const int MID = C.method == 1 ? 2 : 1;
+    d2 ms = B.mid_side[g.own * 2 + face];
+#pragma unroll
+    for (int delta = 0; delta < 2; delta++) {
+        long long cm = delta ? (long long)g.own - g.step : (long long)g.own;             // stencil[MID]
+        long long cm1 = delta ? (long long)g.own - 2 * g.step : (long long)g.own + g.step; // stencil[MID-1]
+        (void)MID;
+        d2 c0 = B.centre[cm], c1 = B.centre[cm1];
+        g.dist[delta] = length(ms - c0);
+        g.d1[delta] = length(c0 - c1);
+    }
+    return g;
+}
+
+// reconstructed face value and slope at velocity v; also the left and right
+// cell values (stencil[MID] / stencil[MID+1] depending on upwinding)
+template <typename T>
+__device__ __forceinline__ void reconstruct(const Consts &C, const void *F, const FaceGeo &g, int v, int delta,
+                                            P2<T> &ff, P2<T> &sg, P2<T> &fl, P2<T> &fr) {
+    const size_t NV = C.NV;
+    const long long sgn = delta ? -1 : 1;   // stencil runs from the upwind side towards the face
+    const long long own = (long long)g.own;
+    // cell of stencil index s: delta=1 -> own + (s-MID-1)*step ; delta=0 -> own + (MID-s)*step
+    if (C.method == 1) {
+        P2<T> s[5];
+#pragma unroll
+        for (int k = 0; k < 5; k++) {
+            long long cell = delta ? own + (k - 3) * g.step : own + (2 - k) * g.step;
+            s[k] = ld<T>(F, (size_t)cell * NV + v);
+        }
+        T dist = (T)g.dist[delta], rdx = (T)(1.0 / g.dist[delta]);
+        sg.x = weno5_T<T>(s[0].x, s[1].x, s[2].x, s[3].x, s[4].x, rdx);
+        sg.y = weno5_T<T>(s[0].y, s[1].y, s[2].y, s[3].y, s[4].y, rdx);
+        ff.x = s[2].x + sg.x * dist;
+        ff.y = s[2].y + sg.y * dist;
+        fl = delta ? s[2] : s[3];
+        fr = delta ? s[3] : s[2];
+    } else {
+        P2<T> s[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            long long cell = delta ? own + (k - 2) * g.step : own + (1 - k) * g.step;
+            s[k] = ld<T>(F, (size_t)cell * NV + v);
+        }
+        T dist = (T)g.dist[delta];
+        if (C.method == 0) {
+            T r1 = (T)(1.0 / g.d1[delta]);
+            sg.x = vanleer_T<T>((s[1].x - s[0].x) * r1, (s[2].x - s[1].x) * r1);
+            sg.y = vanleer_T<T>((s[1].y - s[0].y) * r1, (s[2].y - s[1].y) * r1);
+        } else {
+            T r = (T)(1.0 / g.dist[delta]);
+            sg.x = (s[1].x + T(0.5) * minmod_T<T>(s[2].x - s[1].x, s[1].x - s[0].x) - s[1].x) * r;
+            sg.y = (s[1].y + T(0.5) * minmod_T<T>(s[2].y - s[1].y, s[1].y - s[0].y) - s[1].y) * r;
+        }
+        ff.x = s[1].x + sg.x * dist;
+        ff.y = s[1].y + sg.y * dist;
+        fl = delta ? s[1] : s[2];
+        fr = delta ? s[2] : s[1];
+    }
+    (void)sgn;
+}
+
+// separable Maxwellian exponentials for an axis-aligned normal on a tensor
+// velocity grid: exp(-lam |uv - (U,V)|^2) = Ei[v / nvt] * Ej[v % nvt]
+__device__ inline bool fill_exp_tables(const Consts &C, d2 n, double U, double V, double lam, double *Ei, double *Ej) {
+    if (C.nut == 0 || !(n.x == 0.0 || n.y == 0.0)) return false;
+    const bool wtype = (n.y == 0.0);  // normal along x: uv = (n.x e.x, n.x e.y)
+    for (int k = threadIdx.x; k < C.nut + C.nvt; k += kT) {
+        if (k < C.nut) {
+            double a = wtype ? n.x * C.qu[k] : -n.y * C.qu[k];     // uv.x (W) or uv.y (S) along i
+            double c = wtype ? U : V;
+            Ei[k] = exp(-lam * ((a - c) * (a - c)));
+        } else {
+            int j = k - C.nut;
+            double a = wtype ? n.x * C.qv[j] : n.y * C.qv[j];      // uv.y (W) or uv.x (S) along j
+            double c = wtype ? V : U;
+            Ej[j] = exp(-lam * ((a - c) * (a - c)));
+        }
+    }
+    return true;
+}
+
+// ============================================================== k_face ====
+template <typename T, int NPT>
+__global__ void __launch_bounds__(kT) k_face(const FBlock *__restrict__ blocks, const long long *__restrict__ off,
+                                             int nb, const DtState *__restrict__ ds) {
+    __shared__ double red[kWavesT][16];
+    __shared__ double sc[40];
+    __shared__ double Ei[64], Ej[64];
+    const long long item = blockIdx.x;
+    const int b = find_block(off, nb, item);
+    const FBlock &B = blocks[b];
+    const Consts &C = B.c;
+    const long long loc = item - off[b];
+    int face, mi, mj;
+    if (loc < B.nS) { face = SOUTH; mi = (int)(loc / B.nSj); mj = (int)(loc % B.nSj) + B.ob_S; }
+    else { long long r = loc - B.nS; face = WEST; mi = (int)(r / C.nj) + B.ob_W; mj = (int)(r % C.nj); }
+    const int gi = mi + C.G, gj = mj + C.G;
+    const double dt = ds->dt;
+    const FaceGeo g = face_geo(B, gi, gj, face);
+    const d2 n = B.normal[g.own * 2 + face];
+    const bool wantQ = (face == SOUTH) && (mj < C.nj);
+    const d4 cm = wantQ ? B.macro[(size_t)mi * C.nj + mj] : mk4(0, 0, 0, 1);
+
+    // pass A: reconstruct; face, left and right moments; owner heat flux
+    P2<T> ff[NPT], sg[NPT];
+    double acc[14];
+#pragma unroll
+    for (int q = 0; q < 14; q++) acc[q] = 0.0;
+#pragma unroll
+    for (int k = 0; k < NPT; k++) {
+        const int v = threadIdx.x + kT * k;
+        if (v < C.NV) {
+            d2 uv = face_velocity(C, v, n);
+            P2<T> fl, fr;
+            reconstruct<T>(C, B.f, g, v, upwind(uv.x), ff[k], sg[k], fl, fr);
+            const double u2 = uv.x * uv.x + uv.y * uv.y;
+            double a = (double)ff[k].x, bq = (double)ff[k].y;
+            acc[0] += a; acc[1] += uv.x * a; acc[2] += uv.y * a; acc[3] += 0.5 * (u2 * a + bq);
+            a = (double)fl.x; bq = (double)fl.y;
+            acc[4] += a; acc[5] += uv.x * a; acc[6] += uv.y * a; acc[7] += 0.5 * (u2 * a + bq);
+            a = (double)fr.x; bq = (double)fr.y;
+            acc[8] += a; acc[9] += uv.x * a; acc[10] += uv.y * a; acc[11] += 0.5 * (u2 * a + bq);
+            if (wantQ) {
+                d2 e = C.quad[v];
+                double cx = e.x - cm.s1, cy = e.y - cm.s2, t = (cx * cx + cy * cy) * a + bq / C.K;
+                acc[12] += 0.5 * (cx * t);
+                acc[13] += 0.5 * (cy * t);
+            }
+        }
+    }
+    block_sum<14>(acc, red);
+    if (wantQ && threadIdx.x == 0) B.Q[(size_t)mi * C.nj + mj] = mk2(acc[12], acc[13]);
+
+    // face state, slopes, time slope, time integrals, g0 flux (wave 0)
+    if (threadIdx.x < 64) {
+        d4 W = mk4(acc[0], acc[1], acc[2], acc[3]);
+        d4 WL = mk4(acc[4], acc[5], acc[6], acc[7]), WR = mk4(acc[8], acc[9], acc[10], acc[11]);
+        d4 prim = to_primary(C, W);
+        d2 ms = B.mid_side[g.own * 2 + face];
+        d4 aL = micro_slope(C, prim, (W - WL) / length(ms - B.centre[(long long)g.own - g.step]));
+        d4 aR = micro_slope(C, prim, (WR - W) / length(ms - B.centre[g.own]));
+        Moments m;
+        velocity_moments(C, prim, m);
+        d4 sw = (-prim.s0) * (moment_au(aL, m.MuL, m.Mv, m.Mxi, 1, 0) + moment_au(aR, m.MuR, m.Mv, m.Mxi, 1, 0));
+        d4 aT = micro_slope(C, prim, sw);
+        TimeInt mt = time_integrals(relax_time(C, prim), dt);
+        d4 g0 = prim.s0 * (mt.m0 * moment_uv(m.Mu, m.Mv, m.Mxi, 1, 0, 0) +
+                           mt.m1 * (moment_au(aL, m.MuL, m.Mv, m.Mxi, 2, 0) + moment_au(aR, m.MuR, m.Mv, m.Mxi, 2, 0)) +
+                           mt.m2 * moment_au(aT, m.Mu, m.Mv, m.Mxi, 1, 0));
+        if (threadIdx.x == 0) {
+            double *s = sc;
+            s[0] = prim.s0; s[1] = prim.s1; s[2] = prim.s2; s[3] = prim.s3;
+            s[4] = aL.s0; s[5] = aL.s1; s[6] = aL.s2; s[7] = aL.s3;
+            s[8] = aR.s0; s[9] = aR.s1; s[10] = aR.s2; s[11] = aR.s3;
+            s[12] = aT.s0; s[13] = aT.s1; s[14] = aT.s2; s[15] = aT.s3;
+            s[16] = mt.m0; s[17] = mt.m1; s[18] = mt.m2; s[19] = mt.m3; s[20] = mt.m4;
+            s[21] = m.Mxi[2];
+            s[22] = g0.s0; s[23] = g0.s1; s[24] = g0.s2; s[25] = g0.s3;
+        }
+    }
+    __syncthreads();
+    const d4 prim = mk4(sc[0], sc[1], sc[2], sc[3]);
+    const d4 aL = mk4(sc[4], sc[5], sc[6], sc[7]), aR = mk4(sc[8], sc[9], sc[10], sc[11]);
+    const d4 aT = mk4(sc[12], sc[13], sc[14], sc[15]);
+    const TimeInt mt{sc[16], sc[17], sc[18], sc[19], sc[20]};
+    const double Mxi2 = sc[21];
+    const bool sep = fill_exp_tables(C, n, prim.s1, prim.s2, prim.s3, Ei, Ej);
+
+    // pass B: face heat flux about the face state (calcFaceQ)
+    double q2[2] = {0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < NPT; k++) {
+        const int v = threadIdx.x + kT * k;
+        if (v < C.NV) {
+            d2 uv = face_velocity(C, v, n);
+            double cx = uv.x - prim.s1, cy = uv.y - prim.s2;
+            double t = (cx * cx + cy * cy) * (double)ff[k].x + (double)ff[k].y / C.K;
+            q2[0] += 0.5 * (cx * t);
+            q2[1] += 0.5 * (cy * t);
+        }
+    }
+    block_sum<2>(q2, red);   // also orders the exponential tables before use
+    const d2 Qf = mk2(q2[0], q2[1]);
+
+    // pass C: macro flux sums and the per-velocity flux (macroFlux, distFlux)
+    const double L = B.side_length[g.own * 2 + face];
+    void *fbuf = face == SOUTH ? B.fluxS : B.fluxW;
+    const double rl = prim.s0 * (prim.s3 / kPi);
+    const double sfac = 0.8 * (1 - C.Pr) * (prim.s3 * prim.s3) / prim.s0;
+    double mf[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < NPT; k++) {
+        const int v = threadIdx.x + kT * k;
+        if (v < C.NV) {
+            d2 uv = face_velocity(C, v, n);
+            double cx = uv.x - prim.s1, cy = uv.y - prim.s2, c2 = cx * cx + cy * cy;
+            double E = sep ? Ei[v / C.nvt] * Ej[v % C.nvt] : exp(-prim.s3 * c2);
+            double w = C.weight[v];
+            double f0g = w * (rl * E), f0h = w * ((rl * E) * C.K / (2.0 * prim.s3));
+            double sh = sfac * (cx * Qf.x + cy * Qf.y), part = 2 * prim.s3 * c2;
+            double F0g = sh * (part - 4) * f0g, F0h = sh * (part - 2) * f0h;
+            double fdg = (double)ff[k].x, fdh = (double)ff[k].y, slg = (double)sg[k].x, slh = (double)sg[k].y;
+            double ux = uv.x, uy = uv.y, uu = ux * ux + uy * uy, ux2 = ux * ux;
+            mf[0] += mt.m0 * ux * F0g + mt.m3 * ux * fdg - mt.m4 * ux2 * slg;
+            mf[1] += mt.m0 * ux2 * F0g + mt.m3 * ux2 * fdg - mt.m4 * ux2 * ux * slg;
+            mf[2] += mt.m0 * uy * ux * F0g + mt.m3 * uy * ux * fdg - mt.m4 * uy * ux2 * slg;
+            mf[3] += mt.m0 * 0.5 * (ux * uu * F0g + ux * F0h) + mt.m3 * 0.5 * (ux * uu * fdg + ux * fdh) -
+                     mt.m4 * 0.5 * (ux2 * uu * slg + ux2 * slh);
+            const d4 a = upwind(ux) ? aL : aR;
+            double pg = aT.s0 * f0g + aT.s1 * ux * f0g + aT.s2 * uy * f0g + 0.5 * aT.s3 * (uu * f0g + f0h);
+            double ph = aT.s0 * f0h + aT.s1 * ux * f0h + aT.s2 * uy * f0h + 0.5 * aT.s3 * (uu * f0h + Mxi2 * f0g);
+            double sgv = a.s0 * f0g + a.s1 * ux * f0g + a.s2 * uy * f0g + 0.5 * a.s3 * (uu * f0g + f0h);
+            double shv = a.s0 * f0h + a.s1 * ux * f0h + a.s2 * uy * f0h + 0.5 * a.s3 * (uu * f0h + Mxi2 * f0g);
+            double rg = mt.m0 * ux * (f0g + F0g) + mt.m1 * ux2 * sgv + mt.m2 * ux * pg + mt.m3 * ux * fdg - mt.m4 * ux2 * slg;
+            double rh = mt.m0 * ux * (f0h + F0h) + mt.m1 * ux2 * shv + mt.m2 * ux * ph + mt.m3 * ux * fdh - mt.m4 * ux2 * slh;
+            st<T>(fbuf, g.own * (size_t)C.NV + v, (T)(L * rg), (T)(L * rh));
+        }
+    }
+    block_sum<4>(mf, red);
+    if (threadIdx.x == 0) {
+        d4 o = mk4(mf[0], mf[1], mf[2], mf[3]) + mk4(sc[22], sc[23], sc[24], sc[25]);
+        d2 gl = to_global(mk2(o.s1, o.s2), n);
+        o.s1 = gl.x; o.s2 = gl.y;
+        (face == SOUTH ? B.fmS : B.fmW)[g.own] = L * o;
+    }
+}
+
+// ============================================================== k_wall ====
+struct WallItem { int block, wall; long long start; };
+
+template <typename T, int NPT>
+__global__ void __launch_bounds__(kT) k_wall(const FBlock *__restrict__ blocks, const WallItem *__restrict__ items,
+                                             int nitems, const DtState *__restrict__ ds) {
+    __shared__ double red[kWavesT][16];
+    __shared__ P2<T> wd[NPT * kT];   // reflected distribution (clAdsorb.c:377) for the specular mirror lookup
+    __shared__ double sc[8];
+    const long long item = blockIdx.x;
+    int it = 0;
+    while (it + 1 < nitems && items[it + 1].start <= item) it++;
+    const WallItem W = items[it];
+    const FBlock &B = blocks[W.block];
+    const Consts &C = B.c;
+    const int face = W.wall;
+    const int ci = (int)(item - W.start);
+    int gi, gj, rot, fid;
+    switch (face) {
+    case GNORTH: gi = ci + C.G; gj = C.NJ - C.G; rot = -1; fid = SOUTH; break;
+    case GEAST: gi = C.NI - C.G; gj = ci + C.G; rot = -1; fid = WEST; break;
+    case GSOUTH: gi = ci + C.G; gj = C.G; rot = 1; fid = SOUTH; break;
+    default: gi = C.G; gj = ci + C.G; rot = 1; fid = WEST; break;
+    }
+    const double dt = ds->dt;
+    const FaceGeo g = face_geo(B, gi, gj, fid);
+    const d2 n = B.normal[g.own * 2 + fid];
+    const double L = B.side_length[g.own * 2 + fid];
+    const int mi = gi - C.G, mj = gj - C.G;
+    const bool wantQ = (fid == SOUTH) && (mj < C.nj);
+    const d4 cm = wantQ ? B.macro[(size_t)mi * C.nj + mj] : mk4(0, 0, 0, 1);
+
+    // reconstruction at the wall face (iFace) and the owner heat flux
+    P2<T> ff[NPT];
+    double acc[2] = {0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < NPT; k++) {
+        const int v = threadIdx.x + kT * k;
+        if (v < C.NV) {
+            d2 uv = face_velocity(C, v, n);
+            P2<T> sgk, fl, fr;
+            reconstruct<T>(C, B.f, g, v, upwind(uv.x), ff[k], sgk, fl, fr);
+            if (wantQ) {
+                d2 e = C.quad[v];
+                double a = (double)fr.x, bq = (double)fr.y;
+                double cx = e.x - cm.s1, cy = e.y - cm.s2, t = (cx * cx + cy * cy) * a + bq / C.K;
+                acc[0] += 0.5 * (cx * t);
+                acc[1] += 0.5 * (cy * t);
+            }
+        }
+    }
+    if (wantQ) {
+        block_sum<2>(acc, red);
+        if (threadIdx.x == 0) B.Q[(size_t)mi * C.nj + mj] = mk2(acc[0], acc[1]);
+    }
+    d4 wall = B.wall_prop[(size_t)ci * 4 + face];
+    {
+        d2 wl = to_local(mk2(wall.s1, wall.s2), n);
+        wall.s1 = wl.x; wall.s2 = wl.y;
+    }
+    if (B.bc[face] == 4) {
+        // accommodatingWallDist (clUGKS.c:776-906)
+        double s[2] = {0.0, 0.0};
+#pragma unroll
+        for (int k = 0; k < NPT; k++) {
+            const int v = threadIdx.x + kT * k;
+            if (v < C.NV) {
+                d2 uv = face_velocity(C, v, n);
+                if (leaves_wall(uv.x, rot)) s[1] -= uv.x * maxwell(C, wall, uv, v).x;
+                else s[0] += uv.x * (double)ff[k].x;
+            }
+        }
+        block_sum<2>(s, red);
+        const double ratio = s[0] / s[1];
+#pragma unroll
+        for (int k = 0; k < NPT; k++) {
+            const int v = threadIdx.x + kT * k;
+            if (v < C.NV) {
+                d2 uv = face_velocity(C, v, n);
+                if (leaves_wall(uv.x, rot)) {
+                    d2 m = ratio * maxwell(C, wall, uv, v);
+                    ff[k] = P2<T>{(T)m.x, (T)m.y};
+                }
+            }
+        }
+    } else {
+        // adsorbingWall_P1 (clAdsorb.c:253-482), constant-coverage isotherm
+        d4 cell;
+        switch (face) {
+        case GNORTH: cell = B.macro[(size_t)ci * C.nj + (gj - C.G - 1)]; break;
+        case GEAST: cell = B.macro[(size_t)(gi - C.G - 1) * C.nj + ci]; break;
+        case GSOUTH: cell = B.macro[(size_t)ci * C.nj + (gj - C.G)]; break;
+        default: cell = B.macro[(size_t)(gi - C.G) * C.nj + ci]; break;
+        }
+        const double Tc = 1.0 / cell.s3;
+        const size_t ck = (size_t)ci * 4 + face;
+        const d4 cov0 = B.wall_cover[ck];
+        const double vartheta = cov0.s0, gf = C.gamma_f[face];
+        double s3[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+        for (int k = 0; k < NPT; k++) {
+            const int v = threadIdx.x + kT * k;
+            if (v < C.NV) {
+                d2 uv = face_velocity(C, v, n);
+                int delta = leaves_wall(uv.x, rot);
+                double tg = ((-rot * uv.x) * (1 - delta)) * (double)ff[k].x;
+                double th = ((-rot * uv.x) * (1 - delta)) * (double)ff[k].y;
+                double beta = (1.0 - C.beta_n[face] * exp(-uv.x * uv.x / Tc)) *
+                              (1.0 - C.beta_t[face] * (1.0 - exp(-uv.y * uv.y / Tc)));
+                double ag = beta * tg * gf * (1.0 - vartheta), ah = beta * th * gf * (1.0 - vartheta);
+                int fs = (int)ocl_sign(tg);
+                if ((fs * ag) > (fs * tg)) { ag = tg; ah = th; }
+                s3[0] += tg - ag;
+                s3[1] += ag;
+                s3[2] += beta * tg;
+                wd[threadIdx.x + kT * k] = P2<T>{(T)((tg - ag) / uv.x), (T)((th - ah) / uv.x)};
+            }
+        }
+        block_sum<3>(s3, red);
+        const double r_init = s3[0], a_init = s3[1], adjusted = s3[2];
+        double a_final = a_init, dvtheta = 0.0, des = 0.0;
+        bool early = false;
+        if (gf > 0.0) {
+            if (C.dyn[face] == 1) {
+                dvtheta = dt * C.alpha_p[face] * a_init;
+                if (dvtheta > (1 - vartheta)) {
+                    dvtheta = 1 - vartheta;
+                    a_final = dvtheta / dt * C.alpha_p[face];
+                }
+            }
+            if (!C.cads[face]) {
+                early = true;
+            } else {
+                des = gf * (1.0 / C.adsorb[face] - 1.0) * adjusted * vartheta;
+                if (C.dyn[face] == 1) {
+                    dvtheta -= dt * C.alpha_p[face] * des;
+                    if ((vartheta + dvtheta) < 0.0) des = vartheta / (dt * C.alpha_p[face]);
+                }
+            }
+        }
+        double reflected = cov0.s1, ads_cov = cov0.s2, des_cov = cov0.s3;
+        if (early) {
+            if (threadIdx.x == 0) B.flag[0] = ERR_CLAMPING_ADSORB_ISOTHERM;
+        } else {
+            const double r_final = r_init + a_init - a_final;
+            const double correct = r_final / r_init;
+            if (correct != 1.0) {
+#pragma unroll
+                for (int k = 0; k < NPT; k++) {
+                    P2<T> x = wd[threadIdx.x + kT * k];
+                    wd[threadIdx.x + kT * k] = P2<T>{(T)(correct * (double)x.x), (T)(correct * (double)x.y)};
+                }
+            }
+            reflected = r_final; ads_cov = a_final; des_cov = des;
+            if (threadIdx.x == 0) B.wall_cover[ck] = mk4(vartheta + dvtheta, r_final, a_final, des);
+        }
+        __syncthreads();   // wd complete before the mirror lookups
+        // adsorbingWallS_P2 / D_P2 (clAdsorb.c:656-975)
+        const bool diffuse = B.reflect[face] == 1;
+        double ratio = 1.0;
+        if (reflected > 0.0) {
+            double o[1] = {0.0};
+#pragma unroll
+            for (int k = 0; k < NPT; k++) {
+                const int v = threadIdx.x + kT * k;
+                if (v < C.NV) {
+                    d2 uv = face_velocity(C, v, n);
+                    if (leaves_wall(uv.x, rot)) {
+                        double eg, eh;
+                        if (diffuse) {
+                            d2 m = maxwell(C, wall, uv, v);
+                            eg = m.x; eh = m.y;
+                        } else {
+                            int mv = (int)(fid == WEST ? C.mEW[v] : C.mNS[v]);
+                            P2<T> x = wd[mv];
+                            eg = (double)x.x; eh = (double)x.y;
+                        }
+                        ff[k] = P2<T>{(T)eg, (T)eh};
+                        o[0] += rot * uv.x * eg;
+                    }
+                }
+            }
+            block_sum<1>(o, red);
+            ratio = reflected / o[0];
+        }
+        const int account = ((des_cov == 0.0) && (ads_cov > 0.0) && (reflected == 0.0)) ? 1 : 0;
+        d4 wdes = wall;
+        wdes.s0 = 2 * des_cov * sqrt(kPi * wall.s3);
+        const double keep = 1 - (1 - ratio);
+#pragma unroll
+        for (int k = 0; k < NPT; k++) {
+            const int v = threadIdx.x + kT * k;
+            if (v < C.NV) {
+                d2 uv = face_velocity(C, v, n);
+                if (leaves_wall(uv.x, rot)) {
+                    d2 m = maxwell(C, wdes, uv, v);
+                    double xg = keep * (double)ff[k].x + m.x, xh = keep * (double)ff[k].y + m.y;
+                    if (account) { xg = xg - xg; xh = xh - xh; }
+                    ff[k] = P2<T>{(T)xg, (T)xh};
+                }
+            }
+        }
+    }
+    // wallFlux (clUGKS.c:910-1011) and wallMassEnergyFluxes (:1019-1147)
+    void *fbuf = fid == SOUTH ? B.fluxS : B.fluxW;
+    double m4[12];
+#pragma unroll
+    for (int q = 0; q < 12; q++) m4[q] = 0.0;
+#pragma unroll
+    for (int k = 0; k < NPT; k++) {
+        const int v = threadIdx.x + kT * k;
+        if (v < C.NV) {
+            d2 uv = face_velocity(C, v, n);
+            double wg = (double)ff[k].x, wh = (double)ff[k].y;
+            double Fg = ((uv.x * wg) * L) * dt, Fh = ((uv.x * wh) * L) * dt;
+            st<T>(fbuf, g.own * (size_t)C.NV + v, (T)Fg, (T)Fh);
+            double u2 = uv.x * uv.x + uv.y * uv.y;
+            m4[0] += uv.x * wg;
+            m4[1] += uv.x * uv.x * wg;
+            m4[2] += uv.x * uv.y * wg;
+            m4[3] += 0.5 * uv.x * (u2 * wg + wh);
+            int o = leaves_wall(uv.x, rot);
+            double fgs = (double)(T)Fg, fhs = (double)(T)Fh;
+            m4[4 + o] += fgs;
+            m4[6 + o] += uv.x * fgs;
+            m4[8 + o] += uv.y * fgs;
+            m4[10 + o] += 0.5 * (u2 * fgs + fhs);
+        }
+    }
+    block_sum<12>(m4, red);
+    if (threadIdx.x == 0) {
+        d2 gl = to_global(mk2(m4[1], m4[2]), n);
+        (fid == SOUTH ? B.fmS : B.fmW)[g.own] = (dt * L) * mk4(m4[0], gl.x, gl.y, m4[3]);
+        double *o = B.wall_fluxes + ((size_t)ci * 4 + face) * 8;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            o[2 * q] = (double)(-rot) * m4[4 + 2 * q];
+            o[2 * q + 1] = (double)rot * m4[5 + 2 * q];
+        }
+    }
+    (void)sc;
+}
+
+// ============================================================ k_update ====
+template <typename T, int NPT>
+__global__ void __launch_bounds__(kT) k_update(const FBlock *__restrict__ blocks, const long long *__restrict__ off,
+                                               int nb, const DtState *__restrict__ ds) {
+    __shared__ double tab[4][64];
+    const long long item = blockIdx.x;
+    const int b = find_block(off, nb, item);
+    const FBlock &B = blocks[b];
+    const Consts &C = B.c;
+    const long long cell = item - off[b];
+    const int mi = (int)(cell / C.nj), mj = (int)(cell % C.nj), gi = mi + C.G, gj = mj + C.G;
+    const size_t c0 = (size_t)gi * C.NJ + gj;
+    const double dt = ds->dt;
+    // updateMacro (clUGKS.c:1158-1187)
+    const d4 pold = B.macro[cell];
+    const double A = B.area[c0];
+    const d4 dW = ((B.fmS[c0] - B.fmS[c0 + 1]) + B.fmW[c0]) - B.fmW[c0 + C.NJ];
+    const d4 p = to_primary(C, to_conserved(C, pold) + dW / A);
+    if (threadIdx.x == 0) {
+        B.macro_new[cell] = p;
+        B.residual[cell] = pold;
+    }
+    // UGKS_update (clUGKS.c:1189-1257)
+    const double tau_old = relax_time(C, pold), tau = relax_time(C, p);
+    const d2 Q = B.Q[cell];
+    const bool sep = C.nut > 0;
+    if (sep) {
+        for (int k = threadIdx.x; k < 2 * (C.nut + C.nvt); k += kT) {
+            int which = k / (C.nut + C.nvt), r = k % (C.nut + C.nvt);
+            const d4 &pp = which ? p : pold;
+            double a = r < C.nut ? C.qu[r] - pp.s1 : C.qv[r - C.nut] - pp.s2;
+            tab[2 * which + (r < C.nut ? 0 : 1)][r < C.nut ? r : r - C.nut] = exp(-pp.s3 * (a * a));
+        }
+        __syncthreads();
+    }
+    const double rlo = pold.s0 * (pold.s3 / kPi), rln = p.s0 * (p.s3 / kPi);
+    const double so = 0.8 * (1 - C.Pr) * (pold.s3 * pold.s3) / pold.s0, sn = 0.8 * (1 - C.Pr) * (p.s3 * p.s3) / p.s0;
+    const double hfo = C.K / (2.0 * pold.s3), hfn = C.K / (2.0 * p.s3);
+    const double rA = 1.0 / A, den = 1.0 + 0.5 * dt / tau;
+    bool nan_seen = false;
+#pragma unroll
+    for (int k = 0; k < NPT; k++) {
+        const int v = threadIdx.x + kT * k;
+        if (v < C.NV) {
+            const d2 e = C.quad[v];
+            const double w = C.weight[v];
+            double Eo, En;
+            if (sep) {
+                int iu = v / C.nvt, jv = v % C.nvt;
+                Eo = tab[0][iu] * tab[1][jv];
+                En = tab[2][iu] * tab[3][jv];
+            } else {
+                double ox = e.x - pold.s1, oy = e.y - pold.s2, nx = e.x - p.s1, ny = e.y - p.s2;
+                Eo = exp(-pold.s3 * (ox * ox + oy * oy));
+                En = exp(-p.s3 * (nx * nx + ny * ny));
+            }
+            double ox = e.x - pold.s1, oy = e.y - pold.s2, nx = e.x - p.s1, ny = e.y - p.s2;
+            double gO = w * (rlo * Eo), hO = gO * hfo;
+            double shO = so * (ox * Q.x + oy * Q.y), ptO = 2 * pold.s3 * (ox * ox + oy * oy);
+            double feqOg = gO + shO * (ptO - 4) * gO, feqOh = hO + shO * (ptO - 2) * hO;
+            double gN = w * (rln * En), hN = gN * hfn;
+            double shN = sn * (nx * Q.x + ny * Q.y), ptN = 2 * p.s3 * (nx * nx + ny * ny);
+            double feqNg = gN + shN * (ptN - 4) * gN, feqNh = hN + shN * (ptN - 2) * hN;
+            const size_t kk = c0 * (size_t)C.NV + v;
+            P2<T> fo = ld<T>(B.f, kk);
+            P2<T> sS = ld<T>(B.fluxS, kk), nS = ld<T>(B.fluxS, kk + C.NV);
+            P2<T> wW = ld<T>(B.fluxW, kk), eW = ld<T>(B.fluxW, kk + (size_t)C.NJ * C.NV);
+            double flg = (((double)sS.x - (double)nS.x) + (double)wW.x - (double)eW.x) * rA;
+            double flh = (((double)sS.y - (double)nS.y) + (double)wW.y - (double)eW.y) * rA;
+            double fg = (double)fo.x, fh = (double)fo.y;
+            double ng = (fg + flg + (0.5 * dt) * (feqNg / tau + (feqOg - fg) / tau_old)) / den;
+            double nh = (fh + flh + (0.5 * dt) * (feqNh / tau + (feqOh - fh) / tau_old)) / den;
+            T tg = (T)ng, th = (T)nh;
+            if (isnan(tg) || isnan(th)) nan_seen = true;
+            st<T>(B.f, kk, tg, th);
+        }
+    }
+    if (nan_seen) B.flag[0] = ERR_NAN;
+}
+
+// ============================================================ k_dt =========
+__device__ __forceinline__ unsigned long long dbits(double x) { return (unsigned long long)__double_as_longlong(x); }
+
+__global__ void __launch_bounds__(kT) k_dt(const FBlock *__restrict__ blocks, const long long *__restrict__ off, int nb,
+                                           const DtState *__restrict__ ds, unsigned long long *maxbits) {
+    const long long t = (long long)blockIdx.x * kT + threadIdx.x;
+    const long long total = off[nb];
+    double val = 0.0;
+    int b = -1;
+    if (t < total) {
+        b = find_block(off, nb, t);
+        const FBlock &B = blocks[b];
+        const Consts &C = B.c;
+        const long long cell = t - off[b];
+        const int mi = (int)(cell / C.nj), mj = (int)(cell % C.nj);
+        const d4 m = B.macro_new[cell];
+        B.macro[cell] = m;
+        const int NY = C.NJ + 1, G = C.G;
+        d2 a = B.xy[(size_t)(mi + 1 + G) * NY + mj + G], bq = B.xy[(size_t)(mi + 1 + G) * NY + mj + 1 + G];
+        d2 c = B.xy[(size_t)(mi + G) * NY + mj + 1 + G], d = B.xy[(size_t)(mi + G) * NY + mj + G];
+        double dx = fmax(fmax(fmax(fabs(a.x - bq.x), fabs(c.x - bq.x)), fmax(fabs(c.x - d.x), fabs(a.x - d.x))),
+                         fmax(fabs(c.x - a.x), fabs(bq.x - d.x)));
+        double dy = fmax(fmax(fmax(fabs(a.y - bq.y), fabs(c.y - bq.y)), fmax(fabs(c.y - d.y), fabs(a.y - d.y))),
+                         fmax(fabs(c.y - a.y), fabs(bq.y - d.y)));
+        double sos = sqrt(0.5 * C.gam / m.s3);
+        double lo_u = fmin(C.umax, fabs(m.s1) + sos), lo_v = fmin(C.vmax, fabs(m.s2) + sos);
+        double hi_u = fmax(C.umax, fabs(m.s1) + sos), hi_v = fmax(C.vmax, fabs(m.s2) + sos);
+        double scale = (erf(10.0 * (ds->Kn - 1e-3)) + 1) / 2.0;
+        double u = scale * (hi_u - lo_u) + lo_u, v = scale * (hi_v - lo_v) + lo_v;
+        val = (dx * u + dy * v) / B.area[(size_t)(mi + G) * C.NJ + mj + G];
+        B.time_step[cell] = val;
+    }
+    // positive doubles order like their bit patterns: max via integer max
+    unsigned long long bits = dbits(val);
+    const int b0 = __shfl(b, 0, 64);
+    bool uniform = true;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) uniform &= (__shfl_xor(b, o, 64) == b);
+    uniform = __all(uniform);
+    if (uniform) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            unsigned long long x = __shfl_xor(bits, o, 64);
+            bits = x > bits ? x : bits;
+        }
+        if ((threadIdx.x & 63) == 0 && b0 >= 0) atomicMax(&maxbits[b0], bits);
+    } else if (b >= 0) {
+        atomicMax(&maxbits[b], bits);
+    }
+}
+
+// the global time step (UGKSBlock.getDT + UGKSim.updateTimeStep), applied
+// once at the end of the step and once more when the next step's flux loop
+// re-evaluates it (ugks_sim.py:430-441); time advances by the first value
+__global__ void k_dtfin(DtState *ds, unsigned long long *maxbits, double *max_dt, int nb) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double M = __builtin_inf();
+    for (int b = 0; b < nb; b++) {
+        double mx = __longlong_as_double((long long)maxbits[b]);
+        double md = ds->CFL / mx;
+        if (ds->suggest_dt > 0.0 && ds->suggest_dt < md) md = ds->suggest_dt;
+        max_dt[b] = md;
+        M = fmin(M, md);
+        maxbits[b] = 0ull;
+    }
+    const double lim = 1.0 + ds->delta_dt;
+    double d1 = ds->dt;
+    if (d1 != M) d1 = (M > lim * d1) ? lim * d1 : M;
+    double d2v = d1;
+    if (d2v != M) d2v = (M > lim * d2v) ? lim * d2v : M;
+    ds->time += d1;
+    ds->step += 1;
+    if (d2v != d1 && ds->step == 1) ds->overshoot = 1;
+    ds->dt = d2v;
+}
+
+// ============================================================ k_ghost ======
+struct GhostItem { int block, face; long long start; };
+
+template <typename T>
+__global__ void __launch_bounds__(kT) k_ghost(const FBlock *__restrict__ blocks, const GhostItem *__restrict__ items,
+                                              int nitems) {
+    const long long item = blockIdx.x;
+    int it = 0;
+    while (it + 1 < nitems && items[it + 1].start <= item) it++;
+    const GhostItem GI = items[it];
+    const FBlock &B = blocks[GI.block];
+    const Consts &C = B.c;
+    const int face = GI.face;
+    const long long loc = item - GI.start;
+    const bool ns = (face == GNORTH || face == GSOUTH);
+    const int sj = ns ? C.G : C.nj;
+    const int li = (int)(loc / sj), lj = (int)(loc % sj);
+    int gi, gj;
+    switch (face) {
+    case GNORTH: gi = li + C.G; gj = lj + C.NJ - C.G; break;
+    case GEAST: gi = li + C.NI - C.G; gj = lj + C.G; break;
+    case GSOUTH: gi = li + C.G; gj = lj; break;
+    default: gi = li; gj = lj + C.G; break;
+    }
+    int ei, ej;
+    switch (face) {
+    case GNORTH: ei = gi; ej = C.NJ - C.G - 1; break;
+    case GEAST: ei = C.NI - C.G - 1; ej = gj; break;
+    case GSOUTH: ei = gi; ej = C.G; break;
+    default: ei = C.G; ej = gj; break;
+    }
+    const int ci = ns ? li : lj;
+    const size_t NV = C.NV;
+    const size_t dst = ((size_t)gi * C.NJ + gj) * NV;
+    const int type = B.bc[face];
+    if (type == 0 || type == 5) {           // edgeExchange (clGhostRoutines.c:407-538)
+        const FBlock &O = blocks[B.other[face]];
+        int iB = gi, jB = gj;
+        if (B.flipidx[face]) {
+            if (!ns) jB = (C.NJ - 1) - jB;
+            else iB = (C.NI - 1) - iB;
+        }
+        const int of = B.oface[face], NIB = O.c.NI, NJB = O.c.NJ, G = C.G, NI = C.NI, NJ = C.NJ;
+        // getSwapIndex for the pure shifts (E-W, N-S) and the rotations (clGhostRoutines.c:47-138)
+        auto r90 = [&](int nrot) {
+            int Ni = NI, Nj = NJ;
+            for (int a = 0; a < nrot; a++) { int t = iB; iB = Nj - jB - 1; jB = t; t = Ni; Ni = Nj; Nj = t; }
+        };
+        if (face == GEAST) {
+            if (of == GEAST) { iB = (NI - 1) - iB; iB += NIB - 2 * G; } else if (of == GWEST) iB -= NI - 2 * G;
+            else if (of == GNORTH) { r90(3); jB += NJB - 2 * G; } else { r90(1); jB -= NI - 2 * G; }
+        } else if (face == GWEST) {
+            if (of == GWEST) { iB = (NI - 1) - iB; iB -= NI - 2 * G; } else if (of == GEAST) iB += NIB - 2 * G;
+            else if (of == GNORTH) { r90(1); jB += NJB - 2 * G; } else { r90(3); jB -= NI - 2 * G; }
+        } else if (face == GNORTH) {
+            if (of == GNORTH) { jB = (NJ - 1) - jB; jB += NJB - 2 * G; } else if (of == GSOUTH) jB -= NJ - 2 * G;
+            else if (of == GEAST) { r90(1); iB += NIB - 2 * G; } else { r90(3); iB -= NJ - 2 * G; }
+        } else {
+            if (of == GSOUTH) { jB = (NJ - 1) - jB; jB -= NJ - 2 * G; } else if (of == GNORTH) jB += NJB - 2 * G;
+            else if (of == GEAST) { r90(3); iB += NIB - 2 * G; } else { r90(1); iB -= NJ - 2 * G; }
+        }
+        if (iB < 0 || iB >= NIB || jB < 0 || jB >= NJB) return;
+        const size_t src = ((size_t)iB * NJB + jB) * NV;
+        const int fl = B.flip[face];
+        for (int v = threadIdx.x; v < C.NV; v += kT) {
+            int v2 = v;
+            if (fl == FLIP_NS) v2 = (int)C.mNS[v];
+            else if (fl == FLIP_D) v2 = (int)C.mD[v];
+            else if (fl == FLIP_EW) v2 = (int)C.mEW[v];
+            ((P2<T> *)B.f)[dst + v] = ld<T>(O.f, src + v2);
+        }
+        return;
+    }
+    if (type == 1) {                         // edgeExtrapolate
+        const size_t src = ((size_t)ei * C.NJ + ej) * NV;
+        for (int v = threadIdx.x; v < C.NV; v += kT) ((P2<T> *)B.f)[dst + v] = ld<T>(B.f, src + v);
+        return;
+    }
+    if (type == 3 || type == 9) {            // edgeMirror / edgeBounceBack
+        int mi, mj;
+        switch (face) {
+        case GNORTH: mi = gi; mj = 2 * (C.NJ - C.G) - gj - 1; break;
+        case GEAST: mi = 2 * (C.NI - C.G) - gi - 1; mj = gj; break;
+        case GSOUTH: mi = gi; mj = 2 * C.G - gj - 1; break;
+        default: mi = 2 * C.G - gi - 1; mj = gj; break;
+        }
+        const size_t src = ((size_t)mi * C.NJ + mj) * NV;
+        for (int v = threadIdx.x; v < C.NV; v += kT) {
+            int mv = (int)(type == 9 ? C.mD[v] : (ns ? C.mNS[v] : C.mEW[v]));
+            ((P2<T> *)B.f)[dst + v] = ld<T>(B.f, src + mv);
+        }
+        return;
+    }
+    if (type == 4 || type == 8) {            // edgeConstGrad
+        int ei0 = ei, ej0 = ej;
+        double m;
+        switch (face) {
+        case GNORTH: ej0 = ej - 1; m = (double)(gj - ej); break;
+        case GEAST: ei0 = ei - 1; m = (double)(gi - ei); break;
+        case GSOUTH: ej0 = ej + 1; m = (double)(ej - gj); break;
+        default: ei0 = ei + 1; m = (double)(ei - gi); break;
+        }
+        const size_t s0 = ((size_t)ei0 * C.NJ + ej0) * NV, s1 = ((size_t)ei * C.NJ + ej) * NV;
+        const T mm = (T)m;
+        for (int v = threadIdx.x; v < C.NV; v += kT) {
+            P2<T> f0 = ld<T>(B.f, s0 + v), f1 = ld<T>(B.f, s1 + v);
+            ((P2<T> *)B.f)[dst + v] = P2<T>{f1.x + mm * (f1.x - f0.x), f1.y + mm * (f1.y - f0.y)};
+        }
+        return;
+    }
+    // edgeConstant (2), edgeInflow (6), edgeOutflow (7): a Shakhov equilibrium
+    d4 pr;
+    d2 Q = mk2(0.0, 0.0);
+    const d4 wp = B.wall_prop[(size_t)ci * 4 + face];
+    if (type == 2) {
+        pr = wp;
+    } else {
+        const int noi = (face == GNORTH || face == GEAST) ? gi : ei, noj = (face == GNORTH || face == GEAST) ? gj : ej;
+        Q = B.Q[(size_t)(ei - C.G) * C.nj + (ej - C.G)];
+        pr = B.macro[(size_t)(ei - C.G) * C.nj + (ej - C.G)];
+        d2 l = to_local(mk2(pr.s1, pr.s2), B.normal[((size_t)noi * C.NJ + noj) * 2 + (ns ? SOUTH : WEST)]);
+        pr.s1 = l.x; pr.s2 = l.y;
+        if (type == 6) { pr.s0 = wp.s0; pr.s3 = wp.s3; }
+        else pr.s3 = pr.s0 / (wp.s0 / wp.s3);
+    }
+    for (int v = threadIdx.x; v < C.NV; v += kT) {
+        d2 fe = shakhov_eq(C, pr, Q, C.quad[v], v);
+        ((P2<T> *)B.f)[dst + v] = P2<T>{(T)fe.x, (T)fe.y};
+    }
+}
+
+// FP64 -> FP32 conversion of f (the mixed-precision state starts from the FP64 initial state)
+__global__ void k_to_f32(const double *src, float *dst, long long n) {
+    long long t = (long long)blockIdx.x * kT + threadIdx.x;
+    if (t < n) dst[t] = (float)src[t];
+}
+__global__ void k_to_f64(const float *src, double *dst, long long n) {
+    long long t = (long long)blockIdx.x * kT + threadIdx.x;
+    if (t < n) dst[t] = (double)src[t];
+}
+
+}  // namespace
+
+// ============================================================ host plan ====
+struct ugks_step_plan {
+    int nb, precision, npt;
+    int device;
+    FBlock *d_blocks;
+    long long *d_face_off, *d_cell_off;
+    WallItem *d_walls;
+    GhostItem *d_ghosts;
+    int nwall_items, nghost_items;
+    long long nfaces, nwalls, ncells, nghosts;
+    DtState *d_state;
+    unsigned long long *d_maxbits;
+    double *d_max_dt;
+    std::vector<FBlock> host_blocks;
+};
+
+namespace {
+// ev != nullptr: record ev[0..6] around the six launches (per-kernel timing)
+template <typename T, int NPT> int launch_step(ugks_step_plan *p, hipStream_t s, hipEvent_t *ev) {
+    if (ev) (void)hipEventRecord(ev[0], s);
+    if (p->nfaces > 0)
+        hipLaunchKernelGGL((k_face<T, NPT>), dim3((unsigned)p->nfaces), dim3(kT), 0, s, p->d_blocks, p->d_face_off, p->nb,
+                           p->d_state);
+    if (ev) (void)hipEventRecord(ev[1], s);
+    if (p->nwalls > 0)
+        hipLaunchKernelGGL((k_wall<T, NPT>), dim3((unsigned)p->nwalls), dim3(kT), 0, s, p->d_blocks, p->d_walls,
+                           p->nwall_items, p->d_state);
+    if (ev) (void)hipEventRecord(ev[2], s);
+    hipLaunchKernelGGL((k_update<T, NPT>), dim3((unsigned)p->ncells), dim3(kT), 0, s, p->d_blocks, p->d_cell_off, p->nb,
+                       p->d_state);
+    if (ev) (void)hipEventRecord(ev[3], s);
+    hipLaunchKernelGGL(k_dt, dim3((unsigned)((p->ncells + kT - 1) / kT)), dim3(kT), 0, s, p->d_blocks, p->d_cell_off,
+                       p->nb, p->d_state, p->d_maxbits);
+    if (ev) (void)hipEventRecord(ev[4], s);
+    hipLaunchKernelGGL(k_dtfin, dim3(1), dim3(64), 0, s, p->d_state, p->d_maxbits, p->d_max_dt, p->nb);
+    if (ev) (void)hipEventRecord(ev[5], s);
+    if (p->nghosts > 0)
+        hipLaunchKernelGGL((k_ghost<T>), dim3((unsigned)p->nghosts), dim3(kT), 0, s, p->d_blocks, p->d_ghosts,
+                           p->nghost_items);
+    if (ev) (void)hipEventRecord(ev[6], s);
+    return ugks_rt::check_launch("ugks_step_run");
+}
+
+template <typename T> int dispatch_npt(ugks_step_plan *p, hipStream_t s, hipEvent_t *ev) {
+    switch (p->npt) {
+    case 1: return launch_step<T, 1>(p, s, ev);
+    case 2: return launch_step<T, 2>(p, s, ev);
+    case 3: return launch_step<T, 3>(p, s, ev);
+    case 4: return launch_step<T, 4>(p, s, ev);
+    case 5: return launch_step<T, 5>(p, s, ev);
+    case 6: return launch_step<T, 6>(p, s, ev);
+    case 7: return launch_step<T, 7>(p, s, ev);
+    case 8: return launch_step<T, 8>(p, s, ev);
+    default: return launch_step<T, 10>(p, s, ev);
+    }
+}
+}  // namespace
+
+extern "C" {
+
+int ugks_step_plan_create(ugks_block *const *blocks, const ugks_block_buffers *bufs, const ugks_bc_desc *bcs,
+                          int32_t nblocks, int32_t precision, const ugks_dt_desc *dtd, ugks_step_plan **out) {
+    if (!blocks || !bufs || !bcs || !dtd || !out || nblocks < 1) return ugks_rt::fail(-1, "step plan: bad arguments");
+    if (precision != UGKS_FP64 && precision != UGKS_MIXED) return ugks_rt::fail(-1, "step plan: bad precision");
+    *out = nullptr;
+    ugks_step_plan *p = new ugks_step_plan();
+    p->nb = nblocks;
+    p->precision = precision;
+    int maxnv = 0;
+    std::vector<long long> face_off(nblocks + 1, 0), cell_off(nblocks + 1, 0);
+    std::vector<WallItem> walls;
+    std::vector<GhostItem> ghosts;
+    long long nw = 0, ng = 0;
+    p->host_blocks.resize(nblocks);
+    for (int b = 0; b < nblocks; b++) {
+        if (!blocks[b]) { delete p; return ugks_rt::fail(-1, "step plan: null block"); }
+        const Consts &C = blocks[b]->c;
+        const ugks_block_buffers &u = bufs[b];
+        FBlock &F = p->host_blocks[b];
+        memset(&F, 0, sizeof F);
+        F.c = C;
+        F.f = u.f; F.fluxS = u.flux_f_S; F.fluxW = u.flux_f_W; F.wdist = u.wall_dist;
+        F.fmS = (d4 *)u.flux_macro_S; F.fmW = (d4 *)u.flux_macro_W;
+        F.macro = (d4 *)u.macro; F.macro_new = (d4 *)u.macro_new; F.residual = (d4 *)u.residual;
+        F.wall_cover = (d4 *)u.wall_cover; F.Q = (d2 *)u.Q;
+        F.xy = (const d2 *)u.xy; F.centre = (const d2 *)u.centre; F.mid_side = (const d2 *)u.mid_side;
+        F.normal = (const d2 *)u.normal; F.area = (const double *)u.area; F.side_length = (const double *)u.side_length;
+        F.wall_prop = (const d4 *)u.wall_prop; F.wall_fluxes = (double *)u.wall_fluxes;
+        F.time_step = (double *)u.time_step; F.flag = (double *)u.flag;
+        void *need[] = {u.f, u.flux_f_S, u.flux_f_W, u.flux_macro_S, u.flux_macro_W, u.macro, u.macro_new, u.residual,
+                        u.Q, (void *)u.xy, (void *)u.centre, (void *)u.mid_side, (void *)u.normal, (void *)u.area,
+                        (void *)u.side_length, u.wall_prop, u.wall_cover, u.wall_fluxes, u.time_step, u.flag};
+        for (void *q : need)
+            if (!q) { delete p; return ugks_rt::fail(-1, "step plan: a required buffer is null"); }
+        bool wallf[4];
+        for (int k = 0; k < 4; k++) {
+            const ugks_bc_desc &d = bcs[4 * b + k];
+            F.bc[k] = d.type; F.other[k] = d.other_block; F.oface[k] = d.other_face; F.ori[k] = d.orientation;
+            F.flip[k] = d.flip; F.reflect[k] = d.reflect_type;
+            if (d.type < 0 || d.type > 9) { delete p; return ugks_rt::fail(-1, "step plan: bad BC type"); }
+            if ((d.type == 0 || d.type == 5) && (d.other_block < 0 || d.other_block >= nblocks)) {
+                delete p; return ugks_rt::fail(-1, "step plan: exchange partner out of range");
+            }
+            if (d.type == 8 && d.reflect_type == 2) {
+                delete p; return ugks_rt::fail(-2, "step plan: Cercignani-Lampis re-emission is not implemented");
+            }
+            if (d.type == 8 && !C.cads[k]) {
+                delete p; return ugks_rt::fail(-2, "step plan: tabulated adsorption isotherms are not implemented");
+            }
+            wallf[k] = (d.type == 4 || d.type == 8);
+        }
+        F.ob_S = wallf[GSOUTH] ? 1 : 0;
+        int ot_S = wallf[GNORTH] ? 1 : 0;
+        F.ob_W = wallf[GWEST] ? 1 : 0;
+        int ot_W = wallf[GEAST] ? 1 : 0;
+        F.nSj = C.nj + 1 - F.ob_S - ot_S;
+        F.nS = (int)((long long)C.ni * F.nSj);
+        F.nWi = C.ni + 1 - F.ob_W - ot_W;
+        F.wl = C.ni > C.nj ? C.ni : C.nj;
+        face_off[b + 1] = face_off[b] + F.nS + (long long)F.nWi * C.nj;
+        cell_off[b + 1] = cell_off[b] + (long long)C.ni * C.nj;
+        for (int k = 0; k < 4; k++) {
+            int len = (k == GNORTH || k == GSOUTH) ? C.ni : C.nj;
+            if (wallf[k]) { walls.push_back(WallItem{b, k, nw}); nw += len; }
+            ghosts.push_back(GhostItem{b, k, ng});
+            ng += (long long)len * C.G;
+        }
+        if (C.NV > maxnv) maxnv = C.NV;
+    }
+    // edge-exchange corner check (clGhostRoutines.c:491-512), once per plan
+    for (int b = 0; b < nblocks; b++) {
+        FBlock &F = p->host_blocks[b];
+        for (int k = 0; k < 4; k++) {
+            if (!(F.bc[k] == 0 || F.bc[k] == 5)) continue;
+            const FBlock &O = p->host_blocks[F.other[k]];
+            if (O.c.NV != F.c.NV || O.c.G != F.c.G) {
+                delete p; return ugks_rt::fail(-1, "step plan: joined blocks differ in NV or ghost width");
+            }
+            if (!F.ori[k]) continue;
+            const int G = F.c.G, NI = F.c.NI, NJ = F.c.NJ, NIB = O.c.NI, NJB = O.c.NJ;
+            int A[4], Bv[4];
+            switch (k) {
+            case GNORTH: A[0] = G; A[1] = NJ - G; A[2] = NI - G; A[3] = NJ - G; break;
+            case GEAST: A[0] = NI - G; A[1] = G; A[2] = NI - G; A[3] = NJ - G; break;
+            case GSOUTH: A[0] = G; A[1] = G; A[2] = NI - G; A[3] = G; break;
+            default: A[0] = G; A[1] = G; A[2] = G; A[3] = NJ - G; break;
+            }
+            switch (F.oface[k]) {
+            case GNORTH: Bv[0] = G; Bv[1] = NJB - G; Bv[2] = NIB - G; Bv[3] = NJB - G; break;
+            case GEAST: Bv[0] = NIB - G; Bv[1] = G; Bv[2] = NIB - G; Bv[3] = NJB - G; break;
+            case GSOUTH: Bv[0] = G; Bv[1] = G; Bv[2] = NIB - G; Bv[3] = G; break;
+            default: Bv[0] = G; Bv[1] = G; Bv[2] = G; Bv[3] = NJB - G; break;
+            }
+            d2 va[2], vb[2];
+            int e = 0;
+            e |= ugks_rt::copy_small(&va[0], F.xy + (size_t)A[0] * (NJ + 1) + A[1], sizeof(d2), 0);
+            e |= ugks_rt::copy_small(&va[1], F.xy + (size_t)A[2] * (NJ + 1) + A[3], sizeof(d2), 0);
+            e |= ugks_rt::copy_small(&vb[0], O.xy + (size_t)Bv[0] * (NJB + 1) + Bv[1], sizeof(d2), 0);
+            e |= ugks_rt::copy_small(&vb[1], O.xy + (size_t)Bv[2] * (NJB + 1) + Bv[3], sizeof(d2), 0);
+            if (e) { delete p; return ugks_rt::fail(-1, "step plan: vertex read failed"); }
+            auto ne = [](double a, double c) { return !(fabs(a - c) <= 1e-6); };
+            F.flipidx[k] = ne(va[0].x, vb[0].x) || ne(va[0].y, vb[0].y) || ne(va[1].x, vb[1].x) || ne(va[1].y, vb[1].y);
+        }
+    }
+    int npt = (maxnv + kT - 1) / kT;
+    if (npt > 10) { delete p; return ugks_rt::fail(-2, "step plan: NV above 2560 is not supported by the fused step"); }
+    if (npt == 9) npt = 10;
+    p->npt = npt;
+    for (int b = 0; b < nblocks; b++)
+        if (p->host_blocks[b].c.nut > 64 || p->host_blocks[b].c.nvt > 64) {
+            delete p; return ugks_rt::fail(-2, "step plan: more than 64 velocities per axis");
+        }
+    p->nfaces = face_off[nblocks];
+    p->ncells = cell_off[nblocks];
+    p->nwalls = nw;
+    p->nghosts = ng;
+    p->nwall_items = (int)walls.size();
+    p->nghost_items = (int)ghosts.size();
+    hipError_t e = hipGetDevice(&p->device);
+    size_t bytes_blocks = sizeof(FBlock) * nblocks;
+    auto alloc = [&](void **ptr, size_t n) { return e == hipSuccess ? (e = hipMalloc(ptr, n ? n : 8)) : e; };
+    (void)alloc((void **)&p->d_blocks, bytes_blocks);
+    (void)alloc((void **)&p->d_face_off, sizeof(long long) * (nblocks + 1));
+    (void)alloc((void **)&p->d_cell_off, sizeof(long long) * (nblocks + 1));
+    (void)alloc((void **)&p->d_walls, sizeof(WallItem) * (walls.size() ? walls.size() : 1));
+    (void)alloc((void **)&p->d_ghosts, sizeof(GhostItem) * ghosts.size());
+    (void)alloc((void **)&p->d_state, sizeof(DtState));
+    (void)alloc((void **)&p->d_maxbits, sizeof(unsigned long long) * nblocks);
+    (void)alloc((void **)&p->d_max_dt, sizeof(double) * nblocks);
+    if (e != hipSuccess) { ugks_step_plan_destroy(p); return ugks_rt::hip_fail(e, "step plan: hipMalloc"); }
+    DtState st{dtd->dt, dtd->time, dtd->CFL, dtd->delta_dt, dtd->suggest_dt, dtd->Kn_eff, dtd->step, 0};
+    std::vector<unsigned long long> zeros(nblocks, 0ull);
+    if (e == hipSuccess) e = hipMemcpy(p->d_blocks, p->host_blocks.data(), bytes_blocks, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(p->d_face_off, face_off.data(), sizeof(long long) * (nblocks + 1), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(p->d_cell_off, cell_off.data(), sizeof(long long) * (nblocks + 1), hipMemcpyHostToDevice);
+    if (e == hipSuccess && walls.size())
+        e = hipMemcpy(p->d_walls, walls.data(), sizeof(WallItem) * walls.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(p->d_ghosts, ghosts.data(), sizeof(GhostItem) * ghosts.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(p->d_state, &st, sizeof st, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(p->d_maxbits, zeros.data(), sizeof(unsigned long long) * nblocks, hipMemcpyHostToDevice);
+    if (e != hipSuccess) { ugks_step_plan_destroy(p); return ugks_rt::hip_fail(e, "step plan: upload"); }
+    *out = p;
+    return 0;
+}
+
+int ugks_step_plan_destroy(ugks_step_plan *p) {
+    if (!p) return 0;
+    void *ptrs[] = {p->d_blocks, p->d_face_off, p->d_cell_off, p->d_walls, p->d_ghosts, p->d_state, p->d_maxbits, p->d_max_dt};
+    for (void *q : ptrs)
+        if (q) (void)hipFree(q);
+    delete p;
+    return 0;
+}
+
+int ugks_step_run(ugks_step_plan *p, void *stream, int32_t nsteps) {
+    if (!p || nsteps < 0) return ugks_rt::fail(-1, "step run: bad arguments");
+    hipStream_t s = (hipStream_t)stream;
+    for (int k = 0; k < nsteps; k++) {
+        int rc = p->precision == UGKS_FP64 ? dispatch_npt<double>(p, s, nullptr) : dispatch_npt<float>(p, s, nullptr);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+int ugks_step_run_timed(ugks_step_plan *p, void *stream, int32_t nsteps, float *kernel_ms) {
+    if (!p || nsteps < 0 || !kernel_ms) return ugks_rt::fail(-1, "step run timed: bad arguments");
+    hipStream_t s = (hipStream_t)stream;
+    hipEvent_t ev[7];
+    for (int k = 0; k < 7; k++)
+        if (hipEventCreate(&ev[k]) != hipSuccess) return ugks_rt::fail(-1, "step run timed: event create");
+    for (int k = 0; k < 6; k++) kernel_ms[k] = 0.0f;
+    int rc = 0;
+    for (int it = 0; it < nsteps && !rc; it++) {
+        rc = p->precision == UGKS_FP64 ? dispatch_npt<double>(p, s, ev) : dispatch_npt<float>(p, s, ev);
+        if (rc) break;
+        if (hipEventSynchronize(ev[6]) != hipSuccess) { rc = ugks_rt::fail(-1, "step run timed: sync"); break; }
+        for (int k = 0; k < 6; k++) {
+            float ms = 0.0f;
+            (void)hipEventElapsedTime(&ms, ev[k], ev[k + 1]);
+            kernel_ms[k] += ms;
+        }
+    }
+    for (int k = 0; k < 7; k++) (void)hipEventDestroy(ev[k]);
+    return rc;
+}
+
+int ugks_step_state(ugks_step_plan *p, void *stream, ugks_dt_desc *out, double *max_dt) {
+    if (!p || !out) return ugks_rt::fail(-1, "step state: bad arguments");
+    DtState st;
+    hipError_t e = hipMemcpyAsync(&st, p->d_state, sizeof st, hipMemcpyDeviceToHost, (hipStream_t)stream);
+    if (e == hipSuccess && max_dt)
+        e = hipMemcpyAsync(max_dt, p->d_max_dt, sizeof(double) * p->nb, hipMemcpyDeviceToHost, (hipStream_t)stream);
+    if (e == hipSuccess) e = hipStreamSynchronize((hipStream_t)stream);
+    if (e != hipSuccess) return ugks_rt::hip_fail(e, "step state");
+    out->dt = st.dt; out->time = st.time; out->CFL = st.CFL; out->delta_dt = st.delta_dt;
+    out->suggest_dt = st.suggest_dt; out->Kn_eff = st.Kn; out->step = st.step; out->overshoot = st.overshoot;
+    return 0;
+}
+
+int ugks_step_set_dt(ugks_step_plan *p, void *stream, double dt) {
+    if (!p) return ugks_rt::fail(-1, "set dt: null plan");
+    hipError_t e = hipMemcpyAsync(&p->d_state->dt, &dt, sizeof(double), hipMemcpyHostToDevice, (hipStream_t)stream);
+    if (e == hipSuccess) e = hipStreamSynchronize((hipStream_t)stream);
+    return e == hipSuccess ? 0 : ugks_rt::hip_fail(e, "set dt");
+}
+
+int ugks_f64_to_f32(void *stream, const void *src, void *dst, int64_t n) {
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((n + kT - 1) / kT)), dim3(kT), 0, (hipStream_t)stream,
+                       (const double *)src, (float *)dst, (long long)n);
+    return ugks_rt::check_launch("f64_to_f32");
+}
+
+int ugks_f32_to_f64(void *stream, const void *src, void *dst, int64_t n) {
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(k_to_f64, dim3((unsigned)((n + kT - 1) / kT)), dim3(kT), 0, (hipStream_t)stream,
+                       (const float *)src, (double *)dst, (long long)n);
+    return ugks_rt::check_launch("f32_to_f64");
+}
+
+}  // extern "C"

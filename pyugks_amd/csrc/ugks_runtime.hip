@@ -68,7 +68,15 @@ int ugks_block_create(const ugks_block_desc *d, ugks_block **out) {
     if (d->relax_type < 0 || d->relax_type > 3) return fail(-2, "relax_type must be 0..3");
     if (!d->quad || !d->weight) return fail(-1, "quad and weight tables are required");
     const size_t nv = (size_t)d->NV;
-    const size_t bytes = nv * (2 * sizeof(double) + sizeof(double) + 3 * sizeof(int64_t));
+    // tensor structure of QUAD: v = i*nvt + j with quad[v] = (qu[i], qv[j])
+    size_t nvt = 1;
+    while (nvt < nv && d->quad[2 * nvt] == d->quad[0]) nvt++;
+    size_t nut = nv / nvt;
+    bool tensor = (nut * nvt == nv);
+    for (size_t v = 0; tensor && v < nv; v++)
+        tensor = d->quad[2 * v] == d->quad[2 * (v - v % nvt)] && d->quad[2 * v + 1] == d->quad[2 * (v % nvt) + 1];
+    if (!tensor) nut = nvt = 0;
+    const size_t bytes = nv * (2 * sizeof(double) + sizeof(double) + 3 * sizeof(int64_t)) + (nut + nvt) * sizeof(double);
     ugks_block *b = new ugks_block();
     hipError_t e = hipGetDevice(&b->device);
     if (e != hipSuccess) { delete b; return hip_fail(e, "hipGetDevice"); }
@@ -93,6 +101,9 @@ int ugks_block_create(const ugks_block_desc *d, ugks_block **out) {
         }
         p += nv * sizeof(int64_t);
     }
+    for (size_t i = 0; i < nut; i++) ((double *)p)[i] = d->quad[2 * i * nvt];
+    p += nut * sizeof(double);
+    for (size_t j = 0; j < nvt; j++) ((double *)p)[j] = d->quad[2 * j + 1];
     e = hipMemcpy(b->dev_tables, host, bytes, hipMemcpyHostToDevice);
     free(host);
     if (e != hipSuccess) { (void)hipFree(b->dev_tables); delete b; return hip_fail(e, "hipMemcpy(tables)"); }
@@ -114,7 +125,10 @@ int ugks_block_create(const ugks_block_desc *d, ugks_block **out) {
     c.weight = (const double *)dp; dp += nv * sizeof(double);
     c.mNS = (const int64_t *)dp; dp += nv * sizeof(int64_t);
     c.mEW = (const int64_t *)dp; dp += nv * sizeof(int64_t);
-    c.mD = (const int64_t *)dp;
+    c.mD = (const int64_t *)dp; dp += nv * sizeof(int64_t);
+    c.nut = (int)nut; c.nvt = (int)nvt;
+    c.qu = nut ? (const double *)dp : nullptr; dp += nut * sizeof(double);
+    c.qv = nvt ? (const double *)dp : nullptr;
     *out = b;
     return 0;
 }

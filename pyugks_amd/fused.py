@@ -1,0 +1,111 @@
+"""The fused, batched step (ugks_step_* in include/ugks_hip.h) for a UGKSim.
+
+``StepPlan(sim, precision)`` takes a simulation that the reference-shaped
+path has initialised (geometry, initial f, ghosts, first time step; the
+sequence of ugks_sim.py:52-92) and advances all of its blocks together:
+each step is six launches with the time step kept on the device.  With
+``precision="mixed"`` f and the per-velocity face fluxes are stored and
+computed in FP32; moments, the macroscopic update, heat fluxes, equilibrium
+moments and the time step stay FP64.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import lib as L
+
+
+class StepPlan:
+    def __init__(self, sim, precision: str = "fp64", stream=None):
+        self.sim = sim
+        self.stream = stream
+        self.precision = precision
+        mixed = precision == "mixed"
+        if precision not in ("fp64", "mixed"):
+            raise ValueError("precision must be 'fp64' or 'mixed'")
+        lib = L.load()
+        nb = len(sim.blocks)
+        bufs = (L.BlockBuffers * nb)()
+        bcs = (L.BCDesc * (4 * nb))()
+        handles = (C.c_void_p * nb)()
+        rt = {"S": 0, "D": 1, "CL": 2}
+        for bi, b in enumerate(sim.blocks):
+            b.macro_new_D = L.DeviceArray((b.ni, b.nj, 4), fill=0.0)
+            if mixed:
+                dist = (b.Ni, b.Nj, b.Nv, 2)
+                b.f32_D = L.DeviceArray(dist, np.float32, fill=0.0)
+                b.fluxS32_D = L.DeviceArray(dist, np.float32, fill=0.0)
+                b.fluxW32_D = L.DeviceArray(dist, np.float32, fill=0.0)
+                n = int(np.prod(dist))
+                L.check(lib.ugks_f64_to_f32(stream, b.f_D.ptr, b.f32_D.ptr, n), "f64_to_f32")
+                f, fs, fw = b.f32_D, b.fluxS32_D, b.fluxW32_D
+            else:
+                f, fs, fw = b.f_D, b.flux_f_S_D, b.flux_f_W_D
+            u = bufs[bi]
+            u.f, u.flux_f_S, u.flux_f_W = f.ptr, fs.ptr, fw.ptr
+            u.flux_macro_S, u.flux_macro_W = b.flux_macro_S_D.ptr, b.flux_macro_W_D.ptr
+            u.macro, u.macro_new, u.residual, u.Q = b.macro_D.ptr, b.macro_new_D.ptr, b.residual_D.ptr, b.Q_D.ptr
+            u.xy, u.centre, u.mid_side, u.normal = b.xy_D.ptr, b.centre_D.ptr, b.side_D.ptr, b.normal_D.ptr
+            u.area, u.side_length = b.area_D.ptr, b.length_D.ptr
+            u.wall_prop, u.wall_cover, u.wall_fluxes = b.wall_prop_D.ptr, b.wall_cover_D.ptr, b.wall_fluxes_D.ptr
+            u.wall_dist, u.time_step, u.flag = b.wall_dist_D.ptr, b.time_step_D.ptr, b.flag_D.ptr
+            handles[bi] = b.prg.handle
+            for k, x in enumerate(b.bc_list):
+                bcs[4 * bi + k] = L.BCDesc(int(x.type_of_BC), int(x.other_block), int(x.other_face),
+                                           int(x.orientation), int(x.flip_distribution), rt.get(x.reflect_type, 0))
+        g = sim.gdata
+        dt = L.DtDesc(g.dt, g.time, g.CFL, g.delta_dt, g.suggest_dt if g.suggest_dt else 0.0, g.Kn_eff,
+                      int(sim.step), 0)
+        h = C.c_void_p()
+        L.check(lib.ugks_step_plan_create(handles, bufs, bcs, nb, L.UGKS_MIXED if mixed else L.UGKS_FP64,
+                                          C.byref(dt), C.byref(h)), "ugks_step_plan_create")
+        self.handle = h.value
+        self._keep = (bufs, bcs, handles)
+
+    def run(self, nsteps: int):
+        L.check(L.load().ugks_step_run(self.handle, self.stream, int(nsteps)), "ugks_step_run")
+
+    KERNELS = ("k_face", "k_wall", "k_update", "k_dt", "k_dtfin", "k_ghost")
+
+    def run_timed(self, nsteps: int) -> dict:
+        """Run nsteps with HIP events around each launch; per-kernel total ms."""
+        ms = (C.c_float * 6)()
+        L.check(L.load().ugks_step_run_timed(self.handle, self.stream, int(nsteps), ms), "ugks_step_run_timed")
+        return dict(zip(self.KERNELS, [float(x) for x in ms]))
+
+    def state(self, update_sim: bool = True):
+        d = L.DtDesc()
+        md = np.zeros(len(self.sim.blocks))
+        L.check(L.load().ugks_step_state(self.handle, self.stream, C.byref(d),
+                                         md.ctypes.data_as(C.c_void_p)), "ugks_step_state")
+        if update_sim:
+            g = self.sim.gdata
+            g.dt, g.time, g.step = d.dt, d.time, d.step
+            self.sim.step = d.step
+            for b, m in zip(self.sim.blocks, md):
+                b.max_dt = float(m)
+        return d
+
+    def f64(self, block):
+        """f of one block as FP64 (converted from FP32 for the mixed step)."""
+        if self.precision == "mixed":
+            out = L.DeviceArray(block.f32_D.shape, np.float64)
+            L.check(L.load().ugks_f32_to_f64(self.stream, block.f32_D.ptr, out.ptr, int(np.prod(out.shape))), "f32_to_f64")
+            return out.download(self.stream)
+        return block.f_D.download(self.stream)
+
+    def flag(self):
+        return max(float(b.flag_D.download(self.stream)[0]) for b in self.sim.blocks)
+
+    def close(self):
+        if getattr(self, "handle", None):
+            L.load().ugks_step_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -61,7 +61,35 @@ KERNELS = {
 }
 
 # fused / batched entry points: name -> argtypes (full list)
-FUSED = {}
+FUSED = {
+    "ugks_step_plan_create": [P, P, P, I, I, P, C.POINTER(P)],
+    "ugks_step_plan_destroy": [P],
+    "ugks_step_run": [P, P, I],
+    "ugks_step_run_timed": [P, P, I, P],
+    "ugks_step_state": [P, P, P, P],
+    "ugks_step_set_dt": [P, P, D],
+    "ugks_f64_to_f32": [P, P, P, C.c_int64],
+    "ugks_f32_to_f64": [P, P, P, C.c_int64],
+}
+UGKS_FP64, UGKS_MIXED = 0, 1
+
+
+class BlockBuffers(C.Structure):
+    """ugks_block_buffers (include/ugks_hip.h)"""
+    _fields_ = [(n, P) for n in (
+        "f", "flux_f_S", "flux_f_W", "flux_macro_S", "flux_macro_W", "macro", "macro_new", "residual", "Q",
+        "xy", "centre", "mid_side", "normal", "area", "side_length", "wall_prop", "wall_cover", "wall_fluxes",
+        "wall_dist", "time_step", "flag")]
+
+
+class BCDesc(C.Structure):
+    _fields_ = [("type", I), ("other_block", I), ("other_face", I), ("orientation", I), ("flip", I),
+                ("reflect_type", I)]
+
+
+class DtDesc(C.Structure):
+    _fields_ = [("dt", D), ("time", D), ("CFL", D), ("delta_dt", D), ("suggest_dt", D), ("Kn_eff", D),
+                ("step", I), ("overshoot", I)]
 
 RUNTIME = {
     "ugks_block_create": [P, C.POINTER(P)],
@@ -127,10 +155,9 @@ def load(path: str | None = None):
         f.argtypes = [P, P] + argt
         f.restype = C.c_int
     for name, argt in FUSED.items():
-        if hasattr(lib, name):
-            f = getattr(lib, name)
-            f.argtypes = argt
-            f.restype = C.c_int
+        f = getattr(lib, name)
+        f.argtypes = argt
+        f.restype = C.c_int
     lib.ugks_last_error.restype = C.c_char_p
     lib.ugks_last_error.argtypes = []
     _lib = lib
@@ -162,7 +189,10 @@ class DeviceArray:
         check(lib.ugks_malloc(C.byref(ptr), self.nbytes), "ugks_malloc")
         self.ptr = ptr.value
         if fill is not None:
-            self.upload(np.full(self.shape, fill, dtype=self.dtype))
+            if fill == 0:
+                self.zero()
+            else:
+                self.upload(np.full(self.shape, fill, dtype=self.dtype))
 
     @classmethod
     def from_host(cls, arr, dtype=None):

@@ -33,18 +33,4 @@ def rel_err(a, b, floor=0.0):
     return float(d / scale)
 
 
-def field_errors(macro_a, macro_b, q_a=None, q_b=None):
-    """Relative L-inf and L2 errors of rho, U (vector), T and q."""
-    out = {}
-    ra, rb = macro_a[..., 0], macro_b[..., 0]
-    ua, ub = macro_a[..., 1:3], macro_b[..., 1:3]
-    ta, tb = 1.0 / macro_a[..., 3], 1.0 / macro_b[..., 3]
-    pairs = {"rho": (ra, rb), "U": (ua, ub), "T": (ta, tb)}
-    if q_a is not None:
-        pairs["q"] = (q_a, q_b)
-    for k, (a, b) in pairs.items():
-        d = np.abs(a - b)
-        s = np.max(np.abs(b)) or 1.0
-        out[k + "_linf"] = float(np.max(d) / s)
-        out[k + "_l2"] = float(np.sqrt(np.mean(d ** 2)) / (np.sqrt(np.mean(b ** 2)) or 1.0))
-    return out
+from pyugks_amd.metrics import field_errors  # noqa: E402,F401

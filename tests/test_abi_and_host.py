@@ -1,0 +1,106 @@
+"""CPU checks: the C-ABI library exports what include/ugks_hip.h declares,
+the ctypes descriptors match the header, and the host-side case logic."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from pyugks_amd import bc as bcd
+from pyugks_amd import cases, lib
+from pyugks_amd.data import cartesian_superblock, ghost_coordinates, subdivide_vertex_range, FlowCondition
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    txt = open(os.path.join(ROOT, "include", "ugks_hip.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(?:int|const char \*)\s*(ugks_\w+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = lib.load()
+    names = header_functions()
+    assert len(names) > 40
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    # and the Python binding covers the header exactly
+    assert sorted(set(lib.exported_symbols())) == names
+
+
+def test_descriptor_layouts():
+    # ugks_block_desc: 8 ints, 9 doubles, 5x4 doubles, 4x4 doubles, 2x4 ints, 4 doubles, 5 pointers
+    expect = 8 * 4 + 9 * 8 + 20 * 8 + 16 * 8 + 8 * 4 + 4 * 8 + 5 * 8
+    assert C.sizeof(lib.BlockDesc) == expect
+    assert C.sizeof(lib.BlockBuffers) == 21 * 8
+    assert C.sizeof(lib.BCDesc) == 6 * 4
+    assert C.sizeof(lib.DtDesc) == 6 * 8 + 2 * 4
+
+
+def test_missing_library_fails_loudly(tmp_path, monkeypatch):
+    monkeypatch.setattr(lib, "_lib", None)
+    with pytest.raises(lib.UGKSError):
+        lib.load(str(tmp_path / "nope.so"))
+    monkeypatch.setattr(lib, "_lib", None)
+    lib.load()
+
+
+def test_subdivide_and_superblock_connectivity():
+    assert subdivide_vertex_range(10, 3) == [(0, 3), (3, 6), (6, 10)]
+    blks = cases.cfg2_cavity_2x2(n_per_block=8, nc=8).blocks
+    assert [b.label for b in blks] == ["sblk-0-0", "sblk-0-1", "sblk-1-0", "sblk-1-1"]
+    assert blks[0].bc_list[bcd.EAST].type_of_BC == bcd.ADJACENT and blks[0].bc_list[bcd.EAST].other_block == 2
+    assert blks[0].bc_list[bcd.NORTH].other_block == 1 and blks[0].bc_list[bcd.NORTH].other_face == bcd.SOUTH
+    assert blks[3].bc_list[bcd.WEST].other_block == 1 and blks[3].bc_list[bcd.SOUTH].other_block == 2
+    assert blks[0].bc_list[bcd.WEST].type_of_BC == bcd.DIFFUSE
+
+
+def test_ghost_coordinates_extend_uniform_grid_and_join_blocks():
+    g = cases.cfg2_cavity_2x2(n_per_block=6, nc=8)
+    xys = ghost_coordinates(g.blocks, 3)
+    h = 1.0 / 12
+    for b, xy in zip(g.blocks, xys):
+        x = xy[..., 0]
+        # interior + ghost vertices along i at an interior j are uniformly spaced
+        row = x[:, 5]
+        assert np.allclose(np.diff(row), h, rtol=0, atol=1e-14)
+        col = xy[5, :, 1]
+        assert np.allclose(np.diff(col), h, rtol=0, atol=1e-14)
+    # block (0,0) east ghosts are block (1,0)'s interior vertices, bit for bit
+    assert np.array_equal(xys[0][10:13, 3:10], xys[2][4:7, 3:10])
+
+
+def test_periodic_ghosts_are_translated():
+    gg = cases._base("vanLeer", 0.1)
+    cases.gauss_hermite(gg, 4)
+    bcs = [bcd.PeriodicBC(0, bcd.SOUTH), bcd.PeriodicBC(0, bcd.WEST), bcd.PeriodicBC(0, bcd.NORTH),
+           bcd.PeriodicBC(0, bcd.EAST)]
+    gg.blocks = cartesian_superblock(0.0, 1.0, 0.0, 1.0, 4, 4, bcs, FlowCondition())
+    xy = ghost_coordinates(gg.blocks, 2)[0]
+    assert np.allclose(np.diff(xy[:, 3, 0]), 0.25, atol=1e-14)   # continues past x = 1 and below x = 0
+    assert xy[0, 3, 0] == -0.5 and xy[-1, 3, 0] == 1.5
+
+
+def test_channel_widths_and_adsorbing_mapping():
+    w = cases.channel_widths()
+    assert sum(w) == 1024 and len(w) == 8 and set(w) <= {64, 96, 128, 160, 192}
+    ap = cases.adsorbing_params(0.3, 0.01)
+    theta = ap["adsorb"]
+    assert abs(0.3 * (1 / theta - 1) - 0.01) < 1e-15
+    g = cases.cfg5_channel(widths=[4, 4], height=4, nc=8).prepare()
+    c = g.block_constants(g.blocks[0])
+    assert c["gamma_f"][bcd.SOUTH] == 0.3 and c["alpha_p"][bcd.SOUTH] == 1.0
+    assert c["constant_adsorb"][bcd.SOUTH] == 1 and abs(c["adsorb"][bcd.SOUTH] - 0.967742) < 1e-15
+
+
+def test_batch_cases_are_seeded_and_uneven():
+    a = cases.cfg4_batch()
+    b = cases.cfg4_batch()
+    assert len(a) == 64
+    sides = [x.blocks[0].ni for x in a]
+    assert sides == [x.blocks[0].ni for x in b]
+    assert len(set(sides)) > 3
+    kn = [x.Kn_eff for x in a]
+    assert min(kn) >= 1e-3 and max(kn) <= 1.0

@@ -1,0 +1,121 @@
+"""GPU parity of the fused, batched step against the FP64 oracle.
+
+FP64 plan: f, macro, Q, flux_macro, wall state, dt and time after every
+step within 1e-12 relative (max|a-b| / max|b|; heat fluxes against the
+unit-state scale).  Mixed FP32/FP64 plan: L-inf relative error of rho, U,
+T below 1e-5 after the steps run.
+"""
+import copy
+
+import numpy as np
+import pytest
+
+from helpers import SCALE_FLOOR, field_errors, rel_err
+from pyugks_amd import cases
+from test_gpu_reference_path import CASES
+
+TOL = 1e-12
+pytestmark = pytest.mark.gpu
+
+
+def build(name, precision):
+    from oracle import OracleSim
+    from pyugks_amd.fused import StepPlan
+    from pyugks_amd.sim import UGKSim
+    print(f"[gpu] fused {precision} case {name}", flush=True)
+    gpu = UGKSim(copy.deepcopy(CASES[name]))
+    plan = StepPlan(gpu, precision)
+    ora = OracleSim(copy.deepcopy(CASES[name]), threads=4)
+    return gpu, plan, ora
+
+
+def interior_faces(arr, G, ni, nj):
+    return arr[G:G + ni + 1, G:G + nj + 1]
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_fused_fp64_matches_oracle(hip, name):
+    gpu, plan, ora = build(name, "fp64")
+    for step in range(4):
+        plan.run(1)
+        ora.one_step(True)
+        d = plan.state()
+        assert abs(d.dt - ora.dt) <= TOL * ora.dt, (step, d.dt, ora.dt)
+        assert abs(d.time - ora.time) <= TOL * ora.time
+        assert d.step == ora.sim.step
+        for bi, (bg, bo) in enumerate(zip(gpu.blocks, ora.blocks)):
+            G = bo.consts["ghost"]
+            errs = {
+                "f": rel_err(bg.f_D.download(), bo["f"]),
+                "macro": rel_err(bg.macro_D.download(), bo["macro"]),
+                "Q": rel_err(bg.Q_D.download(), bo["Q"], SCALE_FLOOR["Q"]),
+                "fmS": rel_err(interior_faces(bg.flux_macro_S_D.download(), G, bg.ni, bg.nj),
+                               interior_faces(bo["flux_macro_S"], G, bg.ni, bg.nj)),
+                "fmW": rel_err(interior_faces(bg.flux_macro_W_D.download(), G, bg.ni, bg.nj),
+                               interior_faces(bo["flux_macro_W"], G, bg.ni, bg.nj)),
+                "cover": rel_err(bg.wall_cover_D.download(), bo["wall_cover"]),
+                "wall_fluxes": rel_err(bg.wall_fluxes_D.download(), bo["wall_fluxes"]),
+                "residual": rel_err(bg.residual_D.download(), bo["residual"]),
+            }
+            bad = {k: v for k, v in errs.items() if not v <= TOL}
+            assert not bad, (name, step, bi, bad)
+    assert plan.flag() == 0.0
+
+
+@pytest.mark.parametrize("name", ["cavity_vanleer_gh", "cavity_weno_nc_2x2", "cavity_weno_rho_perturbed",
+                                  "channel_adsorbing"])
+def test_fused_mixed_error_vs_oracle(hip, name):
+    gpu, plan, ora = build(name, "mixed")
+    nsteps = 20
+    plan.run(nsteps)
+    ora.run(nsteps)
+    d = plan.state()
+    assert d.step == ora.sim.step
+    assert abs(d.dt - ora.dt) <= 1e-5 * ora.dt
+    worst = {}
+    for bg, bo in zip(gpu.blocks, ora.blocks):
+        e = field_errors(bg.macro_D.download(), bo["macro"])
+        for k, v in e.items():
+            worst[k] = max(worst.get(k, 0.0), v)
+    print("mixed errors", worst)
+    for k in ("rho_linf", "U_linf", "T_linf"):
+        assert worst[k] <= 1e-5, (name, k, worst[k])
+    assert plan.flag() == 0.0
+
+
+def test_fused_split_2x2_bitwise(hip):
+    from pyugks_amd.fused import StepPlan
+    from pyugks_amd.sim import UGKSim
+    one = UGKSim(cases.cavity(16, ("NC", 8), "WENO5", 0.01, n_cells_j=12))
+    four = UGKSim(cases.cavity(16, ("NC", 8), "WENO5", 0.01, n_cells_j=12, nb=(2, 2)))
+    p1, p4 = StepPlan(one, "fp64"), StepPlan(four, "fp64")
+    p1.run(3)
+    p4.run(3)
+    assert p1.state().dt == p4.state().dt
+    m1 = one.blocks[0].macro()
+    f1 = one.blocks[0].f()
+    G = one.gdata.ghost
+    for b, (i0, j0) in zip(four.blocks, [(0, 0), (0, 6), (8, 0), (8, 6)]):
+        assert np.array_equal(b.macro(), m1[i0:i0 + 8, j0:j0 + 6])
+        assert np.array_equal(b.f()[G:-G, G:-G], f1[G + i0:G + i0 + 8, G + j0:G + j0 + 6])
+
+
+def test_fused_batch_of_cases_matches_separate_plans(hip):
+    """Independent cases in separate plans stepped back to back agree with the
+    oracle; a plan never mixes cases (each case keeps its own time step)."""
+    from oracle import OracleSim
+    from pyugks_amd.fused import StepPlan
+    from pyugks_amd.sim import UGKSim
+    gs = cases.cfg4_batch(ncases=3, sides=(8, 12), vgrids=(8, 12))
+    sims = [UGKSim(copy.deepcopy(g)) for g in gs]
+    plans = [StepPlan(s, "fp64") for s in sims]
+    oras = [OracleSim(copy.deepcopy(g), threads=4) for g in gs]
+    for _ in range(3):
+        for p in plans:
+            p.run(1)
+        for o in oras:
+            o.one_step(True)
+    for s, p, o in zip(sims, plans, oras):
+        assert abs(p.state().dt - o.dt) <= TOL * o.dt
+        assert rel_err(s.blocks[0].macro(), o.blocks[0]["macro"]) <= TOL
+        assert rel_err(s.blocks[0].f(), o.blocks[0]["f"]) <= TOL

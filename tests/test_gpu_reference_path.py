@@ -76,8 +76,12 @@ def test_first_step_all_buffers(hip, name):
     assert abs(gpu.gdata.dt - ora.dt) <= TOL * ora.dt
     worst = {}
     for bi, (bg, bo) in enumerate(zip(gpu.blocks, ora.blocks)):
+        # slopes are difference quotients: where f is locally uniform they are
+        # rounding noise, so they are measured against |f|/h and 1/h
+        h = np.min(bo["side_length"][bo.consts["ghost"]:-bo.consts["ghost"], bo.consts["ghost"]:-bo.consts["ghost"]])
+        floors = dict(SCALE_FLOOR, sigma_D=np.max(np.abs(bo["f"])) / h, aL_D=1.0 / h, aR_D=1.0 / h, aT_D=1.0 / h)
         for gname, oname in BUFFERS.items():
-            e = rel_err(getattr(bg, gname).download(), bo[oname], SCALE_FLOOR.get(gname, 0.0))
+            e = rel_err(getattr(bg, gname).download(), bo[oname], floors.get(gname, 0.0))
             worst[(bi, gname)] = e
     bad = {k: v for k, v in worst.items() if not v <= TOL}
     print("worst", max(worst.values()))
