@@ -183,13 +183,16 @@ int ugks_edgeConstGrad(ugks_block *blk, void *stream, void *Fin, int32_t this_fa
 
 /* ---- fused, batched step (layer 2) -------------------------------------- */
 /* One plan steps any number of blocks (of one or many independent cases) in
- * six launches per step with the time step kept on the device.  It replaces
+ * eight launches per step with the time step kept on the device.  It replaces
  * the sequence UGKSim.one_step drives per block (ugks_sim.py:407-501:
  * UGKS_flux, UGKS_update, calcTimeStep/updateTimeStep, updateAllBC) and
  * produces the same f, macro, Q, flux_macro, wall_cover and wall_fluxes.
  * Blocks given to one plan share one time step (ugks_sim.py:296-308);
  * independent cases that need their own time steps get their own plans. */
 enum { UGKS_FP64 = 0, UGKS_MIXED = 1 };
+/* launches per step, in order: face moments, face state, face flux, wall
+ * faces, cell update, cell time step, time-step finalise, ghost cells */
+#define UGKS_STEP_NKERNELS 8
 
 /* Device buffers of one block, in the layouts of layer 1.  f, flux_f_S,
  * flux_f_W and wall_dist hold (g,h) pairs of double for UGKS_FP64 and of
@@ -223,12 +226,15 @@ int ugks_step_plan_create(ugks_block *const *blocks, const ugks_block_buffers *b
 int ugks_step_plan_destroy(ugks_step_plan *plan);
 /* nsteps full steps: flux, update, time step, BC refresh (ugks_sim.py:407-501) */
 int ugks_step_run(ugks_step_plan *plan, void *stream, int32_t nsteps);
-/* the same steps with HIP events around each launch; kernel_ms[6] receives the
- * summed time of face, wall, update, dt, dt-finalise and ghost launches */
+/* the same steps with HIP events around each launch; kernel_ms[UGKS_STEP_NKERNELS]
+ * receives the summed time of each launch of the step, in the order above */
 int ugks_step_run_timed(ugks_step_plan *plan, void *stream, int32_t nsteps, float *kernel_ms);
 /* read back dt, time, step (and the per-block maximum time steps if max_dt != NULL) */
 int ugks_step_state(ugks_step_plan *plan, void *stream, ugks_dt_desc *out, double *max_dt);
 int ugks_step_set_dt(ugks_step_plan *plan, void *stream, double dt);
+/* plain device-to-device copy kernel of `bytes` (a multiple of 16): the
+ * reference stream for bandwidth and byte-counter calibration */
+int ugks_copy_kernel(void *stream, const void *src, void *dst, int64_t bytes);
 /* element-wise precision conversion of n values (f of the mixed-precision step) */
 int ugks_f64_to_f32(void *stream, const void *src, void *dst, int64_t n);
 int ugks_f32_to_f64(void *stream, const void *src, void *dst, int64_t n);

@@ -1,15 +1,19 @@
 // ugks_fused.hip -- the batched, fused UGKS time step for gfx950.
 //
-// One step of every block of every case in six launches, with no host
+// One step of every block of every case in eight launches, with no host
 // round trip (the reference issues ~30 launches per block per step plus two
 // blocking readbacks, ugks_block.py:1049-1329, :1415-1433):
 //
-//   k_face    one workgroup per interior face (S and W faces of all blocks):
-//             reconstruction (iFace), face/left/right moments
-//             (getFaceCons/getAL/getAR), slopes and time slope
-//             (getPLR/initMacroFlux), face heat flux (calcFaceQ), macro flux
-//             (macroFlux) and the per-velocity flux (distFlux); the owning
-//             cell's heat flux (calcQ) rides along on S faces.
+//   k_faceA   one wavefront per interior face (S and W faces of all blocks):
+//             reconstruction (iFace) and the raw velocity moments of the face
+//             values (getFaceCons and the sums calcFaceQ needs), of the left
+//             and right cells (getAL/getAR) and of the owner cell (calcQ, on
+//             S faces).
+//   k_faceS   one thread per interior face: primary variables, slopes and
+//             time slope (getPLR/initMacroFlux), face heat flux (calcFaceQ)
+//             and the equilibrium part of the macro flux.
+//   k_faceC   one wavefront per interior face: reconstruction again, macro
+//             flux (macroFlux) and the per-velocity flux (distFlux).
 //   k_wall    one workgroup per diffuse/adsorbing wall face: reconstruction,
 //             accommodatingWallDist or adsorbingWall_P1 + S/D_P2, wallFlux,
 //             wallMassEnergyFluxes.
@@ -212,149 +216,205 @@ __device__ inline bool fill_exp_tables(const Consts &C, d2 n, double U, double V
     return true;
 }
 
-// ============================================================== k_face ====
-template <typename T, int NPT>
-__global__ void __launch_bounds__(kT) k_face(const FBlock *__restrict__ blocks, const long long *__restrict__ off,
-                                             int nb, const DtState *__restrict__ ds) {
-    __shared__ double red[kWavesT][16];
-    __shared__ double sc[40];
-    __shared__ double Ei[64], Ej[64];
-    const long long item = blockIdx.x;
-    const int b = find_block(off, nb, item);
+// ============================================================ face passes ==
+// Per interior face: (A) one wavefront reconstructs f at the face for every
+// velocity and sums the raw face moments (the conserved moments of
+// getFaceCons, the third-order ones the face heat flux of calcFaceQ needs,
+// clUGKS.c:522-585), the left/right cell moments (getAL/getAR) and the owner
+// cell's heat flux (calcQ); (S) one thread turns those sums into the face
+// state (getPLR, initMacroFlux, calcFaceQ); (C) one wavefront rebuilds the
+// face values and forms the per-velocity flux (distFlux) and the macro flux
+// (macroFlux).  Face ids: per block, S faces then W faces (walls excluded).
+constexpr int kFS = 24;   // face sums per face
+constexpr int kFT = 32;   // face state per face
+
+struct FaceLoc { int face, mi, mj, gi, gj; };
+__device__ __forceinline__ FaceLoc face_loc(const FBlock &B, long long loc) {
+    const Consts &C = B.c;
+    FaceLoc f;
+    if (loc < B.nS) { f.face = SOUTH; f.mi = (int)(loc / B.nSj); f.mj = (int)(loc % B.nSj) + B.ob_S; }
+    else { long long r = loc - B.nS; f.face = WEST; f.mi = (int)(r / C.nj) + B.ob_W; f.mj = (int)(r % C.nj); }
+    f.gi = f.mi + C.G;
+    f.gj = f.mj + C.G;
+    return f;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kT) k_faceA(const FBlock *__restrict__ blocks, const long long *__restrict__ off,
+                                              int nb, long long nfaces, double *__restrict__ fsum) {
+    const long long fid = (long long)blockIdx.x * kWavesT + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (fid >= nfaces) return;
+    const int lane = threadIdx.x & 63;
+    const int b = find_block(off, nb, fid);
     const FBlock &B = blocks[b];
     const Consts &C = B.c;
-    const long long loc = item - off[b];
-    int face, mi, mj;
-    if (loc < B.nS) { face = SOUTH; mi = (int)(loc / B.nSj); mj = (int)(loc % B.nSj) + B.ob_S; }
-    else { long long r = loc - B.nS; face = WEST; mi = (int)(r / C.nj) + B.ob_W; mj = (int)(r % C.nj); }
-    const int gi = mi + C.G, gj = mj + C.G;
-    const double dt = ds->dt;
-    const FaceGeo g = face_geo(B, gi, gj, face);
-    const d2 n = B.normal[g.own * 2 + face];
-    const bool wantQ = (face == SOUTH) && (mj < C.nj);
-    const d4 cm = wantQ ? B.macro[(size_t)mi * C.nj + mj] : mk4(0, 0, 0, 1);
-
-    // pass A: reconstruct; face, left and right moments; owner heat flux
-    P2<T> ff[NPT], sg[NPT];
-    double acc[14];
+    const FaceLoc L = face_loc(B, fid - off[b]);
+    const FaceGeo g = face_geo(B, L.gi, L.gj, L.face);
+    const d2 n = B.normal[g.own * 2 + L.face];
+    const bool wantQ = (L.face == SOUTH) && (L.mj < C.nj);
+    const d4 cm = wantQ ? B.macro[(size_t)L.mi * C.nj + L.mj] : mk4(0, 0, 0, 1);
+    double a[23];
 #pragma unroll
-    for (int q = 0; q < 14; q++) acc[q] = 0.0;
-#pragma unroll
-    for (int k = 0; k < NPT; k++) {
-        const int v = threadIdx.x + kT * k;
-        if (v < C.NV) {
-            d2 uv = face_velocity(C, v, n);
-            P2<T> fl, fr;
-            reconstruct<T>(C, B.f, g, v, upwind(uv.x), ff[k], sg[k], fl, fr);
-            const double u2 = uv.x * uv.x + uv.y * uv.y;
-            double a = (double)ff[k].x, bq = (double)ff[k].y;
-            acc[0] += a; acc[1] += uv.x * a; acc[2] += uv.y * a; acc[3] += 0.5 * (u2 * a + bq);
-            a = (double)fl.x; bq = (double)fl.y;
-            acc[4] += a; acc[5] += uv.x * a; acc[6] += uv.y * a; acc[7] += 0.5 * (u2 * a + bq);
-            a = (double)fr.x; bq = (double)fr.y;
-            acc[8] += a; acc[9] += uv.x * a; acc[10] += uv.y * a; acc[11] += 0.5 * (u2 * a + bq);
-            if (wantQ) {
-                d2 e = C.quad[v];
-                double cx = e.x - cm.s1, cy = e.y - cm.s2, t = (cx * cx + cy * cy) * a + bq / C.K;
-                acc[12] += 0.5 * (cx * t);
-                acc[13] += 0.5 * (cy * t);
-            }
+    for (int q = 0; q < 23; q++) a[q] = 0.0;
+    for (int v = lane; v < C.NV; v += 64) {
+        const d2 uv = face_velocity(C, v, n);
+        P2<T> ff, sg, fl, fr;
+        reconstruct<T>(C, B.f, g, v, upwind(uv.x), ff, sg, fl, fr);
+        const double u = uv.x, w = uv.y, gx = (double)ff.x, hx = (double)ff.y;
+        const double ug = u * gx, wg = w * gx, u2 = u * u + w * w;
+        a[0] += gx; a[1] += ug; a[2] += wg;
+        a[3] += u * ug; a[4] += w * wg; a[5] += u * wg;
+        a[6] += u * (u * ug); a[7] += w * (w * wg); a[8] += w * (w * ug); a[9] += u * (u * wg);
+        a[10] += hx; a[11] += u * hx; a[12] += w * hx;
+        double x = (double)fl.x, y = (double)fl.y;
+        a[13] += x; a[14] += u * x; a[15] += w * x; a[16] += 0.5 * (u2 * x + y);
+        x = (double)fr.x; y = (double)fr.y;
+        a[17] += x; a[18] += u * x; a[19] += w * x; a[20] += 0.5 * (u2 * x + y);
+        if (wantQ) {
+            const d2 e = C.quad[v];
+            const double cx = e.x - cm.s1, cy = e.y - cm.s2, t = (cx * cx + cy * cy) * x + y / C.K;
+            a[21] += 0.5 * (cx * t);
+            a[22] += 0.5 * (cy * t);
         }
     }
-    block_sum<14>(acc, red);
-    if (wantQ && threadIdx.x == 0) B.Q[(size_t)mi * C.nj + mj] = mk2(acc[12], acc[13]);
-
-    // face state, slopes, time slope, time integrals, g0 flux (wave 0)
-    if (threadIdx.x < 64) {
-        d4 W = mk4(acc[0], acc[1], acc[2], acc[3]);
-        d4 WL = mk4(acc[4], acc[5], acc[6], acc[7]), WR = mk4(acc[8], acc[9], acc[10], acc[11]);
-        d4 prim = to_primary(C, W);
-        d2 ms = B.mid_side[g.own * 2 + face];
-        d4 aL = micro_slope(C, prim, (W - WL) / length(ms - B.centre[(long long)g.own - g.step]));
-        d4 aR = micro_slope(C, prim, (WR - W) / length(ms - B.centre[g.own]));
-        Moments m;
-        velocity_moments(C, prim, m);
-        d4 sw = (-prim.s0) * (moment_au(aL, m.MuL, m.Mv, m.Mxi, 1, 0) + moment_au(aR, m.MuR, m.Mv, m.Mxi, 1, 0));
-        d4 aT = micro_slope(C, prim, sw);
-        TimeInt mt = time_integrals(relax_time(C, prim), dt);
-        d4 g0 = prim.s0 * (mt.m0 * moment_uv(m.Mu, m.Mv, m.Mxi, 1, 0, 0) +
-                           mt.m1 * (moment_au(aL, m.MuL, m.Mv, m.Mxi, 2, 0) + moment_au(aR, m.MuR, m.Mv, m.Mxi, 2, 0)) +
-                           mt.m2 * moment_au(aT, m.Mu, m.Mv, m.Mxi, 1, 0));
-        if (threadIdx.x == 0) {
-            double *s = sc;
-            s[0] = prim.s0; s[1] = prim.s1; s[2] = prim.s2; s[3] = prim.s3;
-            s[4] = aL.s0; s[5] = aL.s1; s[6] = aL.s2; s[7] = aL.s3;
-            s[8] = aR.s0; s[9] = aR.s1; s[10] = aR.s2; s[11] = aR.s3;
-            s[12] = aT.s0; s[13] = aT.s1; s[14] = aT.s2; s[15] = aT.s3;
-            s[16] = mt.m0; s[17] = mt.m1; s[18] = mt.m2; s[19] = mt.m3; s[20] = mt.m4;
-            s[21] = m.Mxi[2];
-            s[22] = g0.s0; s[23] = g0.s1; s[24] = g0.s2; s[25] = g0.s3;
-        }
-    }
-    __syncthreads();
-    const d4 prim = mk4(sc[0], sc[1], sc[2], sc[3]);
-    const d4 aL = mk4(sc[4], sc[5], sc[6], sc[7]), aR = mk4(sc[8], sc[9], sc[10], sc[11]);
-    const d4 aT = mk4(sc[12], sc[13], sc[14], sc[15]);
-    const TimeInt mt{sc[16], sc[17], sc[18], sc[19], sc[20]};
-    const double Mxi2 = sc[21];
-    const bool sep = fill_exp_tables(C, n, prim.s1, prim.s2, prim.s3, Ei, Ej);
-
-    // pass B: face heat flux about the face state (calcFaceQ)
-    double q2[2] = {0.0, 0.0};
 #pragma unroll
-    for (int k = 0; k < NPT; k++) {
-        const int v = threadIdx.x + kT * k;
-        if (v < C.NV) {
-            d2 uv = face_velocity(C, v, n);
-            double cx = uv.x - prim.s1, cy = uv.y - prim.s2;
-            double t = (cx * cx + cy * cy) * (double)ff[k].x + (double)ff[k].y / C.K;
-            q2[0] += 0.5 * (cx * t);
-            q2[1] += 0.5 * (cy * t);
+    for (int q = 0; q < 23; q++) a[q] = wave_sum(a[q]);
+    if (lane == 0) {
+        double *o = fsum + fid * kFS;
+#pragma unroll
+        for (int q = 0; q < 23; q++) o[q] = a[q];
+        if (wantQ) B.Q[(size_t)L.mi * C.nj + L.mj] = mk2(a[21], a[22]);
+    }
+}
+
+// face state: primary variables, slopes, time slope, time integrals, the
+// equilibrium part of the macro flux and the face heat flux (from the raw
+// moments about the face velocity) -- getPLR, initMacroFlux, calcFaceQ
+__global__ void __launch_bounds__(kT) k_faceS(const FBlock *__restrict__ blocks, const long long *__restrict__ off,
+                                              int nb, long long nfaces, const double *__restrict__ fsum,
+                                              double *__restrict__ fstate, const DtState *__restrict__ ds) {
+    const long long fid = (long long)blockIdx.x * kT + threadIdx.x;
+    if (fid >= nfaces) return;
+    const int b = find_block(off, nb, fid);
+    const FBlock &B = blocks[b];
+    const Consts &C = B.c;
+    const FaceLoc L = face_loc(B, fid - off[b]);
+    const size_t own = (size_t)L.gi * C.NJ + L.gj;
+    const long long step = L.face == SOUTH ? 1 : C.NJ;
+    const double *S = fsum + fid * kFS;
+    const d4 W = mk4(S[0], S[1], S[2], 0.5 * ((S[3] + S[4]) + S[10]));
+    const d4 WL = mk4(S[13], S[14], S[15], S[16]), WR = mk4(S[17], S[18], S[19], S[20]);
+    const d4 prim = to_primary(C, W);
+    const d2 ms = B.mid_side[own * 2 + L.face];
+    const d4 aL = micro_slope(C, prim, (W - WL) / length(ms - B.centre[(long long)own - step]));
+    const d4 aR = micro_slope(C, prim, (WR - W) / length(ms - B.centre[own]));
+    Moments m;
+    velocity_moments(C, prim, m);
+    const d4 sw = (-prim.s0) * (moment_au(aL, m.MuL, m.Mv, m.Mxi, 1, 0) + moment_au(aR, m.MuR, m.Mv, m.Mxi, 1, 0));
+    const d4 aT = micro_slope(C, prim, sw);
+    const TimeInt mt = time_integrals(relax_time(C, prim), ds->dt);
+    const d4 g0 = prim.s0 * (mt.m0 * moment_uv(m.Mu, m.Mv, m.Mxi, 1, 0, 0) +
+                             mt.m1 * (moment_au(aL, m.MuL, m.Mv, m.Mxi, 2, 0) + moment_au(aR, m.MuR, m.Mv, m.Mxi, 2, 0)) +
+                             mt.m2 * moment_au(aT, m.Mu, m.Mv, m.Mxi, 1, 0));
+    // face heat flux 1/2 sum (c-U)(|c-U|^2 g + h/K) from the raw moments
+    const double U = prim.s1, V = prim.s2;
+    const double qx = 0.5 * ((S[6] - 3.0 * U * S[3] + 3.0 * U * U * S[1] - U * U * U * S[0]) +
+                             (S[8] - 2.0 * V * S[5] + V * V * S[1] - U * S[4] + 2.0 * U * V * S[2] - U * V * V * S[0]) +
+                             (S[11] - U * S[10]) / C.K);
+    const double qy = 0.5 * ((S[7] - 3.0 * V * S[4] + 3.0 * V * V * S[2] - V * V * V * S[0]) +
+                             (S[9] - 2.0 * U * S[5] + U * U * S[2] - V * S[3] + 2.0 * U * V * S[1] - V * U * U * S[0]) +
+                             (S[12] - V * S[10]) / C.K);
+    const d2 n = B.normal[own * 2 + L.face];
+    double *o = fstate + fid * kFT;
+    o[0] = prim.s0; o[1] = prim.s1; o[2] = prim.s2; o[3] = prim.s3;
+    o[4] = aL.s0; o[5] = aL.s1; o[6] = aL.s2; o[7] = aL.s3;
+    o[8] = aR.s0; o[9] = aR.s1; o[10] = aR.s2; o[11] = aR.s3;
+    o[12] = aT.s0; o[13] = aT.s1; o[14] = aT.s2; o[15] = aT.s3;
+    o[16] = mt.m0; o[17] = mt.m1; o[18] = mt.m2; o[19] = mt.m3; o[20] = mt.m4;
+    o[21] = m.Mxi[2]; o[22] = qx; o[23] = qy;
+    o[24] = g0.s0; o[25] = g0.s1; o[26] = g0.s2; o[27] = g0.s3;
+    o[28] = B.side_length[own * 2 + L.face]; o[29] = n.x; o[30] = n.y; o[31] = 0.0;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kT) k_faceC(const FBlock *__restrict__ blocks, const long long *__restrict__ off,
+                                              int nb, long long nfaces, const double *__restrict__ fstate) {
+    const long long fid = (long long)blockIdx.x * kWavesT + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (fid >= nfaces) return;
+    const int lane = threadIdx.x & 63;
+    const int b = find_block(off, nb, fid);
+    const FBlock &B = blocks[b];
+    const Consts &C = B.c;
+    const FaceLoc L = face_loc(B, fid - off[b]);
+    const FaceGeo g = face_geo(B, L.gi, L.gj, L.face);
+    const double *sv = fstate + fid * kFT;
+    const d4 prim = mk4(sv[0], sv[1], sv[2], sv[3]);
+    const d4 aL = mk4(sv[4], sv[5], sv[6], sv[7]), aR = mk4(sv[8], sv[9], sv[10], sv[11]);
+    const d4 aT = mk4(sv[12], sv[13], sv[14], sv[15]);
+    const TimeInt mt{sv[16], sv[17], sv[18], sv[19], sv[20]};
+    const double Mxi2 = sv[21];
+    const d2 Qf = mk2(sv[22], sv[23]);
+    const double Lf = sv[28];
+    const d2 n = mk2(sv[29], sv[30]);
+    // separable Maxwellian: lane i holds exp(-lam (a_i - c)^2) of axis 1, lane j of axis 2
+    const bool sep = C.nut > 0 && (n.x == 0.0 || n.y == 0.0);
+    double Ei = 0.0, Ej = 0.0;
+    if (sep) {
+        const bool wtype = (n.y == 0.0);
+        if (lane < C.nut) {
+            const double a = wtype ? n.x * C.qu[lane] : -n.y * C.qu[lane], c = wtype ? prim.s1 : prim.s2;
+            Ei = exp(-prim.s3 * ((a - c) * (a - c)));
+        }
+        if (lane < C.nvt) {
+            const double a = wtype ? n.x * C.qv[lane] : n.y * C.qv[lane], c = wtype ? prim.s2 : prim.s1;
+            Ej = exp(-prim.s3 * ((a - c) * (a - c)));
         }
     }
-    block_sum<2>(q2, red);   // also orders the exponential tables before use
-    const d2 Qf = mk2(q2[0], q2[1]);
-
-    // pass C: macro flux sums and the per-velocity flux (macroFlux, distFlux)
-    const double L = B.side_length[g.own * 2 + face];
-    void *fbuf = face == SOUTH ? B.fluxS : B.fluxW;
+    void *fbuf = L.face == SOUTH ? B.fluxS : B.fluxW;
     const double rl = prim.s0 * (prim.s3 / kPi);
+    const double hk = C.K / (2.0 * prim.s3);
     const double sfac = 0.8 * (1 - C.Pr) * (prim.s3 * prim.s3) / prim.s0;
     double mf[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-    for (int k = 0; k < NPT; k++) {
-        const int v = threadIdx.x + kT * k;
-        if (v < C.NV) {
-            d2 uv = face_velocity(C, v, n);
-            double cx = uv.x - prim.s1, cy = uv.y - prim.s2, c2 = cx * cx + cy * cy;
-            double E = sep ? Ei[v / C.nvt] * Ej[v % C.nvt] : exp(-prim.s3 * c2);
-            double w = C.weight[v];
-            double f0g = w * (rl * E), f0h = w * ((rl * E) * C.K / (2.0 * prim.s3));
-            double sh = sfac * (cx * Qf.x + cy * Qf.y), part = 2 * prim.s3 * c2;
-            double F0g = sh * (part - 4) * f0g, F0h = sh * (part - 2) * f0h;
-            double fdg = (double)ff[k].x, fdh = (double)ff[k].y, slg = (double)sg[k].x, slh = (double)sg[k].y;
-            double ux = uv.x, uy = uv.y, uu = ux * ux + uy * uy, ux2 = ux * ux;
-            mf[0] += mt.m0 * ux * F0g + mt.m3 * ux * fdg - mt.m4 * ux2 * slg;
-            mf[1] += mt.m0 * ux2 * F0g + mt.m3 * ux2 * fdg - mt.m4 * ux2 * ux * slg;
-            mf[2] += mt.m0 * uy * ux * F0g + mt.m3 * uy * ux * fdg - mt.m4 * uy * ux2 * slg;
-            mf[3] += mt.m0 * 0.5 * (ux * uu * F0g + ux * F0h) + mt.m3 * 0.5 * (ux * uu * fdg + ux * fdh) -
-                     mt.m4 * 0.5 * (ux2 * uu * slg + ux2 * slh);
-            const d4 a = upwind(ux) ? aL : aR;
-            double pg = aT.s0 * f0g + aT.s1 * ux * f0g + aT.s2 * uy * f0g + 0.5 * aT.s3 * (uu * f0g + f0h);
-            double ph = aT.s0 * f0h + aT.s1 * ux * f0h + aT.s2 * uy * f0h + 0.5 * aT.s3 * (uu * f0h + Mxi2 * f0g);
-            double sgv = a.s0 * f0g + a.s1 * ux * f0g + a.s2 * uy * f0g + 0.5 * a.s3 * (uu * f0g + f0h);
-            double shv = a.s0 * f0h + a.s1 * ux * f0h + a.s2 * uy * f0h + 0.5 * a.s3 * (uu * f0h + Mxi2 * f0g);
-            double rg = mt.m0 * ux * (f0g + F0g) + mt.m1 * ux2 * sgv + mt.m2 * ux * pg + mt.m3 * ux * fdg - mt.m4 * ux2 * slg;
-            double rh = mt.m0 * ux * (f0h + F0h) + mt.m1 * ux2 * shv + mt.m2 * ux * ph + mt.m3 * ux * fdh - mt.m4 * ux2 * slh;
-            st<T>(fbuf, g.own * (size_t)C.NV + v, (T)(L * rg), (T)(L * rh));
-        }
+    const int nchunk = (C.NV + 63) >> 6;
+    for (int ck = 0; ck < nchunk; ck++) {
+        const int v = ck * 64 + lane;
+        const bool act = v < C.NV;
+        const int vv = act ? v : 0;
+        double E = 0.0;
+        if (sep) E = __shfl(Ei, vv / C.nvt, 64) * __shfl(Ej, vv % C.nvt, 64);
+        if (!act) continue;
+        const d2 uv = face_velocity(C, v, n);
+        P2<T> ff, sg, fl, fr;
+        reconstruct<T>(C, B.f, g, v, upwind(uv.x), ff, sg, fl, fr);
+        const double cx = uv.x - prim.s1, cy = uv.y - prim.s2, c2 = cx * cx + cy * cy;
+        if (!sep) E = exp(-prim.s3 * c2);
+        const double w = C.weight[v];
+        const double f0g = w * (rl * E), f0h = w * ((rl * E) * hk);
+        const double sh = sfac * (cx * Qf.x + cy * Qf.y), part = 2 * prim.s3 * c2;
+        const double F0g = sh * (part - 4) * f0g, F0h = sh * (part - 2) * f0h;
+        const double fdg = (double)ff.x, fdh = (double)ff.y, slg = (double)sg.x, slh = (double)sg.y;
+        const double ux = uv.x, uy = uv.y, uu = ux * ux + uy * uy, ux2 = ux * ux;
+        mf[0] += mt.m0 * ux * F0g + mt.m3 * ux * fdg - mt.m4 * ux2 * slg;
+        mf[1] += mt.m0 * ux2 * F0g + mt.m3 * ux2 * fdg - mt.m4 * ux2 * ux * slg;
+        mf[2] += mt.m0 * uy * ux * F0g + mt.m3 * uy * ux * fdg - mt.m4 * uy * ux2 * slg;
+        mf[3] += mt.m0 * 0.5 * (ux * uu * F0g + ux * F0h) + mt.m3 * 0.5 * (ux * uu * fdg + ux * fdh) -
+                 mt.m4 * 0.5 * (ux2 * uu * slg + ux2 * slh);
+        const d4 a = upwind(ux) ? aL : aR;
+        const double pg = aT.s0 * f0g + aT.s1 * ux * f0g + aT.s2 * uy * f0g + 0.5 * aT.s3 * (uu * f0g + f0h);
+        const double ph = aT.s0 * f0h + aT.s1 * ux * f0h + aT.s2 * uy * f0h + 0.5 * aT.s3 * (uu * f0h + Mxi2 * f0g);
+        const double sgv = a.s0 * f0g + a.s1 * ux * f0g + a.s2 * uy * f0g + 0.5 * a.s3 * (uu * f0g + f0h);
+        const double shv = a.s0 * f0h + a.s1 * ux * f0h + a.s2 * uy * f0h + 0.5 * a.s3 * (uu * f0h + Mxi2 * f0g);
+        const double rg = mt.m0 * ux * (f0g + F0g) + mt.m1 * ux2 * sgv + mt.m2 * ux * pg + mt.m3 * ux * fdg - mt.m4 * ux2 * slg;
+        const double rh = mt.m0 * ux * (f0h + F0h) + mt.m1 * ux2 * shv + mt.m2 * ux * ph + mt.m3 * ux * fdh - mt.m4 * ux2 * slh;
+        st<T>(fbuf, g.own * (size_t)C.NV + v, (T)(Lf * rg), (T)(Lf * rh));
     }
-    block_sum<4>(mf, red);
-    if (threadIdx.x == 0) {
-        d4 o = mk4(mf[0], mf[1], mf[2], mf[3]) + mk4(sc[22], sc[23], sc[24], sc[25]);
-        d2 gl = to_global(mk2(o.s1, o.s2), n);
+#pragma unroll
+    for (int q = 0; q < 4; q++) mf[q] = wave_sum(mf[q]);
+    if (lane == 0) {
+        d4 o = mk4(mf[0], mf[1], mf[2], mf[3]) + mk4(sv[24], sv[25], sv[26], sv[27]);
+        const d2 gl = to_global(mk2(o.s1, o.s2), n);
         o.s1 = gl.x; o.s2 = gl.y;
-        (face == SOUTH ? B.fmS : B.fmW)[g.own] = L * o;
+        (L.face == SOUTH ? B.fmS : B.fmW)[g.own] = Lf * o;
     }
 }
 
@@ -898,6 +958,11 @@ __global__ void k_to_f64(const float *src, double *dst, long long n) {
     if (t < n) dst[t] = (double)src[t];
 }
 
+// plain streaming copy (16 B per lane per iteration, grid-stride)
+__global__ void __launch_bounds__(kT) k_copy(const double2 *__restrict__ src, double2 *__restrict__ dst, long long n) {
+    for (long long t = (long long)blockIdx.x * kT + threadIdx.x; t < n; t += (long long)gridDim.x * kT) dst[t] = src[t];
+}
+
 }  // namespace
 
 // ============================================================ host plan ====
@@ -911,35 +976,47 @@ struct ugks_step_plan {
     int nwall_items, nghost_items;
     long long nfaces, nwalls, ncells, nghosts;
     DtState *d_state;
+    double *d_fsum, *d_fstate;          // per interior face: raw sums, face state
     unsigned long long *d_maxbits;
     double *d_max_dt;
     std::vector<FBlock> host_blocks;
 };
 
 namespace {
-// ev != nullptr: record ev[0..6] around the six launches (per-kernel timing)
+// ev != nullptr: record ev[0..kLaunches] around the launches (per-kernel timing)
+constexpr int kLaunches = UGKS_STEP_NKERNELS;
 template <typename T, int NPT> int launch_step(ugks_step_plan *p, hipStream_t s, hipEvent_t *ev) {
+    const unsigned gw = (unsigned)((p->nfaces + kWavesT - 1) / kWavesT);   // one wavefront per face
+    const unsigned gt = (unsigned)((p->nfaces + kT - 1) / kT);             // one thread per face
     if (ev) (void)hipEventRecord(ev[0], s);
     if (p->nfaces > 0)
-        hipLaunchKernelGGL((k_face<T, NPT>), dim3((unsigned)p->nfaces), dim3(kT), 0, s, p->d_blocks, p->d_face_off, p->nb,
-                           p->d_state);
+        hipLaunchKernelGGL((k_faceA<T>), dim3(gw), dim3(kT), 0, s, p->d_blocks, p->d_face_off, p->nb, p->nfaces,
+                           p->d_fsum);
     if (ev) (void)hipEventRecord(ev[1], s);
+    if (p->nfaces > 0)
+        hipLaunchKernelGGL(k_faceS, dim3(gt), dim3(kT), 0, s, p->d_blocks, p->d_face_off, p->nb, p->nfaces,
+                           (const double *)p->d_fsum, p->d_fstate, (const DtState *)p->d_state);
+    if (ev) (void)hipEventRecord(ev[2], s);
+    if (p->nfaces > 0)
+        hipLaunchKernelGGL((k_faceC<T>), dim3(gw), dim3(kT), 0, s, p->d_blocks, p->d_face_off, p->nb, p->nfaces,
+                           (const double *)p->d_fstate);
+    if (ev) (void)hipEventRecord(ev[3], s);
     if (p->nwalls > 0)
         hipLaunchKernelGGL((k_wall<T, NPT>), dim3((unsigned)p->nwalls), dim3(kT), 0, s, p->d_blocks, p->d_walls,
                            p->nwall_items, p->d_state);
-    if (ev) (void)hipEventRecord(ev[2], s);
+    if (ev) (void)hipEventRecord(ev[4], s);
     hipLaunchKernelGGL((k_update<T, NPT>), dim3((unsigned)p->ncells), dim3(kT), 0, s, p->d_blocks, p->d_cell_off, p->nb,
                        p->d_state);
-    if (ev) (void)hipEventRecord(ev[3], s);
+    if (ev) (void)hipEventRecord(ev[5], s);
     hipLaunchKernelGGL(k_dt, dim3((unsigned)((p->ncells + kT - 1) / kT)), dim3(kT), 0, s, p->d_blocks, p->d_cell_off,
                        p->nb, p->d_state, p->d_maxbits);
-    if (ev) (void)hipEventRecord(ev[4], s);
+    if (ev) (void)hipEventRecord(ev[6], s);
     hipLaunchKernelGGL(k_dtfin, dim3(1), dim3(64), 0, s, p->d_state, p->d_maxbits, p->d_max_dt, p->nb);
-    if (ev) (void)hipEventRecord(ev[5], s);
+    if (ev) (void)hipEventRecord(ev[7], s);
     if (p->nghosts > 0)
         hipLaunchKernelGGL((k_ghost<T>), dim3((unsigned)p->nghosts), dim3(kT), 0, s, p->d_blocks, p->d_ghosts,
                            p->nghost_items);
-    if (ev) (void)hipEventRecord(ev[6], s);
+    if (ev) (void)hipEventRecord(ev[8], s);
     return ugks_rt::check_launch("ugks_step_run");
 }
 
@@ -1089,6 +1166,8 @@ int ugks_step_plan_create(ugks_block *const *blocks, const ugks_block_buffers *b
     (void)alloc((void **)&p->d_state, sizeof(DtState));
     (void)alloc((void **)&p->d_maxbits, sizeof(unsigned long long) * nblocks);
     (void)alloc((void **)&p->d_max_dt, sizeof(double) * nblocks);
+    (void)alloc((void **)&p->d_fsum, sizeof(double) * kFS * (size_t)p->nfaces);
+    (void)alloc((void **)&p->d_fstate, sizeof(double) * kFT * (size_t)p->nfaces);
     if (e != hipSuccess) { ugks_step_plan_destroy(p); return ugks_rt::hip_fail(e, "step plan: hipMalloc"); }
     DtState st{dtd->dt, dtd->time, dtd->CFL, dtd->delta_dt, dtd->suggest_dt, dtd->Kn_eff, dtd->step, 0};
     std::vector<unsigned long long> zeros(nblocks, 0ull);
@@ -1107,7 +1186,8 @@ int ugks_step_plan_create(ugks_block *const *blocks, const ugks_block_buffers *b
 
 int ugks_step_plan_destroy(ugks_step_plan *p) {
     if (!p) return 0;
-    void *ptrs[] = {p->d_blocks, p->d_face_off, p->d_cell_off, p->d_walls, p->d_ghosts, p->d_state, p->d_maxbits, p->d_max_dt};
+    void *ptrs[] = {p->d_blocks, p->d_face_off, p->d_cell_off, p->d_walls, p->d_ghosts, p->d_state,
+                    p->d_maxbits, p->d_max_dt, p->d_fsum, p->d_fstate};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     delete p;
@@ -1127,22 +1207,25 @@ int ugks_step_run(ugks_step_plan *p, void *stream, int32_t nsteps) {
 int ugks_step_run_timed(ugks_step_plan *p, void *stream, int32_t nsteps, float *kernel_ms) {
     if (!p || nsteps < 0 || !kernel_ms) return ugks_rt::fail(-1, "step run timed: bad arguments");
     hipStream_t s = (hipStream_t)stream;
-    hipEvent_t ev[7];
-    for (int k = 0; k < 7; k++)
-        if (hipEventCreate(&ev[k]) != hipSuccess) return ugks_rt::fail(-1, "step run timed: event create");
-    for (int k = 0; k < 6; k++) kernel_ms[k] = 0.0f;
+    hipEvent_t ev[kLaunches + 1];
+    for (int k = 0; k <= kLaunches; k++)
+        if (hipEventCreate(&ev[k]) != hipSuccess) {
+            for (int j = 0; j < k; j++) (void)hipEventDestroy(ev[j]);
+            return ugks_rt::fail(-1, "step run timed: event create");
+        }
+    for (int k = 0; k < kLaunches; k++) kernel_ms[k] = 0.0f;
     int rc = 0;
     for (int it = 0; it < nsteps && !rc; it++) {
         rc = p->precision == UGKS_FP64 ? dispatch_npt<double>(p, s, ev) : dispatch_npt<float>(p, s, ev);
         if (rc) break;
-        if (hipEventSynchronize(ev[6]) != hipSuccess) { rc = ugks_rt::fail(-1, "step run timed: sync"); break; }
-        for (int k = 0; k < 6; k++) {
+        if (hipEventSynchronize(ev[kLaunches]) != hipSuccess) { rc = ugks_rt::fail(-1, "step run timed: sync"); break; }
+        for (int k = 0; k < kLaunches; k++) {
             float ms = 0.0f;
             (void)hipEventElapsedTime(&ms, ev[k], ev[k + 1]);
             kernel_ms[k] += ms;
         }
     }
-    for (int k = 0; k < 7; k++) (void)hipEventDestroy(ev[k]);
+    for (int k = 0; k <= kLaunches; k++) (void)hipEventDestroy(ev[k]);
     return rc;
 }
 
@@ -1164,6 +1247,15 @@ int ugks_step_set_dt(ugks_step_plan *p, void *stream, double dt) {
     hipError_t e = hipMemcpyAsync(&p->d_state->dt, &dt, sizeof(double), hipMemcpyHostToDevice, (hipStream_t)stream);
     if (e == hipSuccess) e = hipStreamSynchronize((hipStream_t)stream);
     return e == hipSuccess ? 0 : ugks_rt::hip_fail(e, "set dt");
+}
+
+int ugks_copy_kernel(void *stream, const void *src, void *dst, int64_t bytes) {
+    if (bytes <= 0 || bytes % 16) return ugks_rt::fail(-1, "copy: size must be a positive multiple of 16");
+    long long n = bytes / 16;
+    unsigned grid = (unsigned)((n + kT - 1) / kT);
+    if (grid > 8192) grid = 8192;
+    hipLaunchKernelGGL(k_copy, dim3(grid), dim3(kT), 0, (hipStream_t)stream, (const double2 *)src, (double2 *)dst, n);
+    return ugks_rt::check_launch("copy");
 }
 
 int ugks_f64_to_f32(void *stream, const void *src, void *dst, int64_t n) {

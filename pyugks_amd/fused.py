@@ -67,11 +67,11 @@ class StepPlan:
     def run(self, nsteps: int):
         L.check(L.load().ugks_step_run(self.handle, self.stream, int(nsteps)), "ugks_step_run")
 
-    KERNELS = ("k_face", "k_wall", "k_update", "k_dt", "k_dtfin", "k_ghost")
+    KERNELS = ("k_faceA", "k_faceS", "k_faceC", "k_wall", "k_update", "k_dt", "k_dtfin", "k_ghost")
 
     def run_timed(self, nsteps: int) -> dict:
         """Run nsteps with HIP events around each launch; per-kernel total ms."""
-        ms = (C.c_float * 6)()
+        ms = (C.c_float * len(self.KERNELS))()
         L.check(L.load().ugks_step_run_timed(self.handle, self.stream, int(nsteps), ms), "ugks_step_run_timed")
         return dict(zip(self.KERNELS, [float(x) for x in ms]))
 

@@ -68,6 +68,7 @@ FUSED = {
     "ugks_step_run_timed": [P, P, I, P],
     "ugks_step_state": [P, P, P, P],
     "ugks_step_set_dt": [P, P, D],
+    "ugks_copy_kernel": [P, P, P, C.c_int64],
     "ugks_f64_to_f32": [P, P, P, C.c_int64],
     "ugks_f32_to_f64": [P, P, P, C.c_int64],
 }
@@ -273,6 +274,25 @@ class Program:
             self.close()
         except Exception:
             pass
+
+
+class CLCompatProgram(Program):
+    """The reference's call shape, unchanged: ``prg.<kernel>(queue, global_size,
+    local_size, *args)`` (ugks_block.py:1066-1295).  ``queue`` is a HIP stream
+    handle (or None); the NDRange sizes are accepted and ignored, because each
+    kernel derives its launch from the block and its face/offset arguments;
+    buffers are DeviceArray objects or raw device pointers."""
+
+    def __getattr__(self, name):
+        if name not in KERNELS:
+            raise AttributeError(name)
+        f = getattr(load(), "ugks_" + name)
+
+        def call(queue, global_size, local_size, *args):
+            return check(f(self.handle, queue, *[_arg(a) for a in args]), name)
+
+        call.__name__ = name
+        return call
 
 
 def make_desc(consts: dict, tables) -> tuple:
