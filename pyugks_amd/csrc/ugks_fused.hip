@@ -46,6 +46,21 @@ namespace {
 constexpr int kT = 256;           // threads per workgroup
 constexpr int kWavesT = kT / 64;
 
+// wavefront-per-face kernels: workgroup size and an optional occupancy hint
+#ifndef UGKS_FACE_WG
+#define UGKS_FACE_WG 256
+#endif
+#ifndef UGKS_FACE_WPE
+#define UGKS_FACE_WPE 0
+#endif
+constexpr int kFW = UGKS_FACE_WG;
+constexpr int kFWaves = kFW / 64;
+#if UGKS_FACE_WPE
+#define FACE_KERNEL __global__ void __launch_bounds__(kFW) __attribute__((amdgpu_waves_per_eu(UGKS_FACE_WPE)))
+#else
+#define FACE_KERNEL __global__ void __launch_bounds__(kFW)
+#endif
+
 template <typename T> struct __align__(2 * sizeof(T)) P2 { T x, y; };
 
 // device record of one block for the fused kernels
@@ -93,19 +108,56 @@ template <int N> __device__ __forceinline__ void block_sum(double (&a)[N], doubl
 }
 
 // ---- reconstruction (iFace, clUGKS.c:134-195) in precision T ---------------
-template <typename T> __device__ __forceinline__ T weno5_T(T s0, T s1, T s2, T s3, T s4, T rdx) {
-    T t1 = s0 - T(2) * s1 + s2, t2 = s0 - T(4) * s1 + T(3) * s2;
-    T b0 = T(13.0 / 12.0) * t1 * t1 + T(0.25) * t2 * t2;
-    t1 = s1 - T(2) * s2 + s3; t2 = s1 - s3;
-    T b1 = T(13.0 / 12.0) * t1 * t1 + T(0.25) * t2 * t2;
-    t1 = s2 - T(2) * s3 + s4; t2 = T(3) * s2 - T(4) * s3 + s4;
-    T b2 = T(13.0 / 12.0) * t1 * t1 + T(0.25) * t2 * t2;
-    T e0 = T(1e-16) + b0, e1 = T(1e-16) + b1, e2 = T(1e-16) + b2;
-    T a0 = T(0.1) / (e0 * e0), a1 = T(0.6) / (e1 * e1), a2 = T(0.3) / (e2 * e2);
-    T f0 = T(1. / 3.) * s0 - T(7. / 6.) * s1 + T(11. / 6.) * s2;
-    T f1 = -T(1. / 6.) * s1 + T(5. / 6.) * s2 + T(1. / 3.) * s3;
-    T f2 = T(1. / 3.) * s2 + T(5. / 6.) * s3 - T(1. / 6.) * s4;
-    T fL = (a0 * f0 + a1 * f1 + a2 * f2) / (a0 + a1 + a2);
+// quotient a / b from the hardware reciprocal estimate refined by two
+// Newton steps and one residual correction (within an ulp or two of the
+// correctly rounded quotient; b is a positive normal number here)
+__device__ __forceinline__ double qdiv(double a, double b) {
+    double r = __builtin_amdgcn_rcp(b);
+    double e = fma(-b, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-b, r, 1.0);
+    r = fma(r, e, r);
+    double q = a * r;
+    return fma(r, fma(-b, q, a), q);
+}
+__device__ __forceinline__ float rcp_T(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// WENO5 slope.  The three nonlinear weights a_k = d_k / (eps + b_k)^2 enter
+// only through sum(a_k f_k) / sum(a_k); in double both sums are scaled by
+// prod_k (eps + b_k)^2 so that a single division remains ((eps + b_k)^2 >=
+// 1e-32, the products stay far from underflow), and the linear weights d_k
+// are folded into the candidate stencils.  In float that product can
+// underflow, so the weights use reciprocal estimates instead.
+__device__ __forceinline__ double weno5_T(double s0, double s1, double s2, double s3, double s4, double rdx) {
+    double t1 = s0 - 2.0 * s1 + s2, t2 = s0 - 4.0 * s1 + 3.0 * s2;
+    const double b0 = (13.0 / 12.0) * t1 * t1 + 0.25 * t2 * t2;
+    t1 = s1 - 2.0 * s2 + s3; t2 = s1 - s3;
+    const double b1 = (13.0 / 12.0) * t1 * t1 + 0.25 * t2 * t2;
+    t1 = s2 - 2.0 * s3 + s4; t2 = 3.0 * s2 - 4.0 * s3 + s4;
+    const double b2 = (13.0 / 12.0) * t1 * t1 + 0.25 * t2 * t2;
+    const double e0 = 1e-16 + b0, e1 = 1e-16 + b1, e2 = 1e-16 + b2;
+    const double q0 = e0 * e0, q1 = e1 * e1, q2 = e2 * e2;
+    const double p0 = q1 * q2, p1 = q0 * q2, p2 = q0 * q1;
+    // d_k f_k with d = (0.1, 0.6, 0.3)
+    const double f0 = (0.1 / 3.) * s0 - (0.7 / 6.) * s1 + (1.1 / 6.) * s2;
+    const double f1 = -0.1 * s1 + 0.5 * s2 + 0.2 * s3;
+    const double f2 = 0.1 * s2 + 0.25 * s3 - 0.05 * s4;
+    const double fL = qdiv(p0 * f0 + p1 * f1 + p2 * f2, 0.1 * p0 + 0.6 * p1 + 0.3 * p2);
+    return (fL - s2) * rdx;
+}
+__device__ __forceinline__ float weno5_T(float s0, float s1, float s2, float s3, float s4, float rdx) {
+    float t1 = s0 - 2.f * s1 + s2, t2 = s0 - 4.f * s1 + 3.f * s2;
+    const float b0 = (13.f / 12.f) * t1 * t1 + 0.25f * t2 * t2;
+    t1 = s1 - 2.f * s2 + s3; t2 = s1 - s3;
+    const float b1 = (13.f / 12.f) * t1 * t1 + 0.25f * t2 * t2;
+    t1 = s2 - 2.f * s3 + s4; t2 = 3.f * s2 - 4.f * s3 + s4;
+    const float b2 = (13.f / 12.f) * t1 * t1 + 0.25f * t2 * t2;
+    const float e0 = 1e-16f + b0, e1 = 1e-16f + b1, e2 = 1e-16f + b2;
+    const float a0 = 0.1f * rcp_T(e0 * e0), a1 = 0.6f * rcp_T(e1 * e1), a2 = 0.3f * rcp_T(e2 * e2);
+    const float f0 = (1.f / 3.f) * s0 - (7.f / 6.f) * s1 + (11.f / 6.f) * s2;
+    const float f1 = -(1.f / 6.f) * s1 + (5.f / 6.f) * s2 + (1.f / 3.f) * s3;
+    const float f2 = (1.f / 3.f) * s2 + (5.f / 6.f) * s3 - (1.f / 6.f) * s4;
+    const float fL = (a0 * f0 + a1 * f1 + a2 * f2) * rcp_T(a0 + a1 + a2);
     return (fL - s2) * rdx;
 }
 template <typename T> __device__ __forceinline__ T vanleer_T(T s1, T s2) {
@@ -121,12 +173,16 @@ template <typename T> __device__ __forceinline__ T minmod_T(T a, T b) {
     return T(0);
 }
 
-// per-face geometry of both upwind choices
+#define AS1 __attribute__((address_space(1)))
+
+// per-face geometry of both upwind choices (delta = 1: upwind cell below the
+// face, stencil runs upwards; delta = 0: from above, downwards)
 struct FaceGeo {
-    size_t own;       // owner cell (gi, gj) linear index
-    long long step;   // cells per unit step along the normal (1 for S, NJ for W)
-    double dist[2];   // |midside - c[MID]| for delta = 0, 1
-    double d1[2];     // vanLeer divisor |c[MID] - c[MID-1]| for delta = 0, 1
+    size_t own;         // owner cell (gi, gj) linear index
+    long long step;     // cells per unit step along the normal (1 for S, NJ for W)
+    double dist[2];     // |midside - c[MID]|
+    double rdist[2];    // 1 / dist
+    double rd1[2];      // 1 / |c[MID] - c[MID-1]| (vanLeer divisor)
 };
 
 __device__ inline FaceGeo face_geo(const FBlock &B, int gi, int gj, int face) {
@@ -134,86 +190,84 @@ __device__ inline FaceGeo face_geo(const FBlock &B, int gi, int gj, int face) {
     FaceGeo g;
     g.own = (size_t)gi * C.NJ + gj;
     g.step = face == SOUTH ? 1 : C.NJ;
-    const int MID = C.method == 1 ? 2 : 1;
     d2 ms = B.mid_side[g.own * 2 + face];
 #pragma unroll
     for (int delta = 0; delta < 2; delta++) {
         long long cm = delta ? (long long)g.own - g.step : (long long)g.own;             // stencil[MID]
         long long cm1 = delta ? (long long)g.own - 2 * g.step : (long long)g.own + g.step; // stencil[MID-1]
-        (void)MID;
         d2 c0 = B.centre[cm], c1 = B.centre[cm1];
         g.dist[delta] = length(ms - c0);
-        g.d1[delta] = length(c0 - c1);
+        g.rdist[delta] = 1.0 / g.dist[delta];
+        g.rd1[delta] = 1.0 / length(c0 - c1);
     }
     return g;
 }
 
-// reconstructed face value and slope at velocity v; also the left and right
-// cell values (stencil[MID] / stencil[MID+1] depending on upwinding)
+// Stencil addressing: one (wave-uniform) base pointer at the lowest stencil
+// cell, own - R*step (R = 3 for WENO5, 2 otherwise), and 32-bit byte offsets
+// rel*sb + v*sizeof(P2) per lane; delta = 1 takes rel = k, delta = 0 rel =
+// 2R-1-k for stencil index k.
+template <typename T> struct Stencil {
+    const AS1 char *base;
+    unsigned sb;
+};
+template <typename T> __device__ __forceinline__ Stencil<T> stencil(const Consts &C, const void *F, const FaceGeo &g) {
+    const int R = C.method == 1 ? 3 : 2;
+    Stencil<T> st;
+    st.base = (const AS1 char *)F + ((long long)g.own - R * g.step) * (long long)C.NV * (long long)sizeof(P2<T>);
+    st.sb = (unsigned)(g.step * C.NV * (long long)sizeof(P2<T>));
+    return st;
+}
+template <typename T> using V2 = T __attribute__((ext_vector_type(2)));
+template <typename T> __device__ __forceinline__ P2<T> sld(const Stencil<T> &st, unsigned rel, unsigned vb) {
+    const V2<T> x = *(const AS1 V2<T> *)(st.base + (rel * st.sb + vb));
+    return P2<T>{x.x, x.y};
+}
+
+// reconstructed face value and slope at velocity v (iFace); fl / fr, the
+// cells below / above the face, are read by the callers that need them
 template <typename T>
-__device__ __forceinline__ void reconstruct(const Consts &C, const void *F, const FaceGeo &g, int v, int delta,
-                                            P2<T> &ff, P2<T> &sg, P2<T> &fl, P2<T> &fr) {
-    const size_t NV = C.NV;
-    const long long sgn = delta ? -1 : 1;   // stencil runs from the upwind side towards the face
-    const long long own = (long long)g.own;
-    // cell of stencil index s: delta=1 -> own + (s-MID-1)*step ; delta=0 -> own + (MID-s)*step
+__device__ __forceinline__ void reconstruct(const Consts &C, const Stencil<T> &st, const FaceGeo &g, int v, int delta,
+                                            P2<T> &ff, P2<T> &sg, P2<T> *fl = nullptr, P2<T> *fr = nullptr) {
+    const unsigned vb = (unsigned)v * (unsigned)sizeof(P2<T>);
     if (C.method == 1) {
         P2<T> s[5];
 #pragma unroll
-        for (int k = 0; k < 5; k++) {
-            long long cell = delta ? own + (k - 3) * g.step : own + (2 - k) * g.step;
-            s[k] = ld<T>(F, (size_t)cell * NV + v);
-        }
-        T dist = (T)g.dist[delta], rdx = (T)(1.0 / g.dist[delta]);
-        sg.x = weno5_T<T>(s[0].x, s[1].x, s[2].x, s[3].x, s[4].x, rdx);
-        sg.y = weno5_T<T>(s[0].y, s[1].y, s[2].y, s[3].y, s[4].y, rdx);
+        for (int k = 0; k < 5; k++) s[k] = sld<T>(st, delta ? k : 5 - k, vb);
+        const T dist = (T)(delta ? g.dist[1] : g.dist[0]), rdx = (T)(delta ? g.rdist[1] : g.rdist[0]);
+        sg.x = weno5_T(s[0].x, s[1].x, s[2].x, s[3].x, s[4].x, rdx);
+        sg.y = weno5_T(s[0].y, s[1].y, s[2].y, s[3].y, s[4].y, rdx);
         ff.x = s[2].x + sg.x * dist;
         ff.y = s[2].y + sg.y * dist;
-        fl = delta ? s[2] : s[3];
-        fr = delta ? s[3] : s[2];
+        if (fl) *fl = delta ? s[2] : s[3];
+        if (fr) *fr = delta ? s[3] : s[2];
     } else {
         P2<T> s[3];
 #pragma unroll
-        for (int k = 0; k < 3; k++) {
-            long long cell = delta ? own + (k - 2) * g.step : own + (1 - k) * g.step;
-            s[k] = ld<T>(F, (size_t)cell * NV + v);
-        }
-        T dist = (T)g.dist[delta];
+        for (int k = 0; k < 3; k++) s[k] = sld<T>(st, delta ? k : 3 - k, vb);
+        const T dist = (T)(delta ? g.dist[1] : g.dist[0]);
         if (C.method == 0) {
-            T r1 = (T)(1.0 / g.d1[delta]);
+            const T r1 = (T)(delta ? g.rd1[1] : g.rd1[0]);
             sg.x = vanleer_T<T>((s[1].x - s[0].x) * r1, (s[2].x - s[1].x) * r1);
             sg.y = vanleer_T<T>((s[1].y - s[0].y) * r1, (s[2].y - s[1].y) * r1);
         } else {
-            T r = (T)(1.0 / g.dist[delta]);
+            const T r = (T)(delta ? g.rdist[1] : g.rdist[0]);
             sg.x = (s[1].x + T(0.5) * minmod_T<T>(s[2].x - s[1].x, s[1].x - s[0].x) - s[1].x) * r;
             sg.y = (s[1].y + T(0.5) * minmod_T<T>(s[2].y - s[1].y, s[1].y - s[0].y) - s[1].y) * r;
         }
         ff.x = s[1].x + sg.x * dist;
         ff.y = s[1].y + sg.y * dist;
-        fl = delta ? s[1] : s[2];
-        fr = delta ? s[2] : s[1];
+        if (fl) *fl = delta ? s[1] : s[2];
+        if (fr) *fr = delta ? s[2] : s[1];
     }
-    (void)sgn;
 }
-
-// separable Maxwellian exponentials for an axis-aligned normal on a tensor
-// velocity grid: exp(-lam |uv - (U,V)|^2) = Ei[v / nvt] * Ej[v % nvt]
-__device__ inline bool fill_exp_tables(const Consts &C, d2 n, double U, double V, double lam, double *Ei, double *Ej) {
-    if (C.nut == 0 || !(n.x == 0.0 || n.y == 0.0)) return false;
-    const bool wtype = (n.y == 0.0);  // normal along x: uv = (n.x e.x, n.x e.y)
-    for (int k = threadIdx.x; k < C.nut + C.nvt; k += kT) {
-        if (k < C.nut) {
-            double a = wtype ? n.x * C.qu[k] : -n.y * C.qu[k];     // uv.x (W) or uv.y (S) along i
-            double c = wtype ? U : V;
-            Ei[k] = exp(-lam * ((a - c) * (a - c)));
-        } else {
-            int j = k - C.nut;
-            double a = wtype ? n.x * C.qv[j] : n.y * C.qv[j];      // uv.y (W) or uv.x (S) along j
-            double c = wtype ? V : U;
-            Ej[j] = exp(-lam * ((a - c) * (a - c)));
-        }
-    }
-    return true;
+// the cells below (rel R-1) and above (rel R, the owner) the face
+template <typename T>
+__device__ __forceinline__ void face_cells(const Consts &C, const Stencil<T> &st, int v, P2<T> &fl, P2<T> &fr) {
+    const unsigned vb = (unsigned)v * (unsigned)sizeof(P2<T>);
+    const unsigned R = C.method == 1 ? 3 : 2;
+    fl = sld<T>(st, R - 1, vb);
+    fr = sld<T>(st, R, vb);
 }
 
 // ============================================================ face passes ==
@@ -240,9 +294,9 @@ __device__ __forceinline__ FaceLoc face_loc(const FBlock &B, long long loc) {
 }
 
 template <typename T>
-__global__ void __launch_bounds__(kT) k_faceA(const FBlock *__restrict__ blocks, const long long *__restrict__ off,
+FACE_KERNEL k_faceA(const FBlock *__restrict__ blocks, const long long *__restrict__ off,
                                               int nb, long long nfaces, double *__restrict__ fsum) {
-    const long long fid = (long long)blockIdx.x * kWavesT + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const long long fid = (long long)blockIdx.x * kFWaves + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (fid >= nfaces) return;
     const int lane = threadIdx.x & 63;
     const int b = find_block(off, nb, fid);
@@ -250,21 +304,28 @@ __global__ void __launch_bounds__(kT) k_faceA(const FBlock *__restrict__ blocks,
     const Consts &C = B.c;
     const FaceLoc L = face_loc(B, fid - off[b]);
     const FaceGeo g = face_geo(B, L.gi, L.gj, L.face);
+    const Stencil<T> stn = stencil<T>(C, B.f, g);
     const d2 n = B.normal[g.own * 2 + L.face];
     const bool wantQ = (L.face == SOUTH) && (L.mj < C.nj);
     const d4 cm = wantQ ? B.macro[(size_t)L.mi * C.nj + L.mj] : mk4(0, 0, 0, 1);
+    const double rK = 1.0 / C.K;
     double a[23];
 #pragma unroll
     for (int q = 0; q < 23; q++) a[q] = 0.0;
     for (int v = lane; v < C.NV; v += 64) {
         const d2 uv = face_velocity(C, v, n);
         P2<T> ff, sg, fl, fr;
-        reconstruct<T>(C, B.f, g, v, upwind(uv.x), ff, sg, fl, fr);
-        const double u = uv.x, w = uv.y, gx = (double)ff.x, hx = (double)ff.y;
-        const double ug = u * gx, wg = w * gx, u2 = u * u + w * w;
+#if UGKS_FACE_LR_LOAD
+        reconstruct<T>(C, stn, g, v, upwind(uv.x), ff, sg);
+        face_cells<T>(C, stn, v, fl, fr);
+#else
+        reconstruct<T>(C, stn, g, v, upwind(uv.x), ff, sg, &fl, &fr);
+#endif
+        const double u = uv.x, w = uv.y, gx = (double)ff.x, hx = (double)ff.y, u2 = u * u + w * w;
+        const double ug = u * gx, wg = w * gx, uug = u * ug, wwg = w * wg;
         a[0] += gx; a[1] += ug; a[2] += wg;
-        a[3] += u * ug; a[4] += w * wg; a[5] += u * wg;
-        a[6] += u * (u * ug); a[7] += w * (w * wg); a[8] += w * (w * ug); a[9] += u * (u * wg);
+        a[3] += uug; a[4] += wwg; a[5] += u * wg;
+        a[6] += u * uug; a[7] += w * wwg; a[8] += u * wwg; a[9] += w * uug;
         a[10] += hx; a[11] += u * hx; a[12] += w * hx;
         double x = (double)fl.x, y = (double)fl.y;
         a[13] += x; a[14] += u * x; a[15] += w * x; a[16] += 0.5 * (u2 * x + y);
@@ -272,7 +333,7 @@ __global__ void __launch_bounds__(kT) k_faceA(const FBlock *__restrict__ blocks,
         a[17] += x; a[18] += u * x; a[19] += w * x; a[20] += 0.5 * (u2 * x + y);
         if (wantQ) {
             const d2 e = C.quad[v];
-            const double cx = e.x - cm.s1, cy = e.y - cm.s2, t = (cx * cx + cy * cy) * x + y / C.K;
+            const double cx = e.x - cm.s1, cy = e.y - cm.s2, t = (cx * cx + cy * cy) * x + y * rK;
             a[21] += 0.5 * (cx * t);
             a[22] += 0.5 * (cy * t);
         }
@@ -337,9 +398,11 @@ __global__ void __launch_bounds__(kT) k_faceS(const FBlock *__restrict__ blocks,
 }
 
 template <typename T>
-__global__ void __launch_bounds__(kT) k_faceC(const FBlock *__restrict__ blocks, const long long *__restrict__ off,
+FACE_KERNEL k_faceC(const FBlock *__restrict__ blocks, const long long *__restrict__ off,
                                               int nb, long long nfaces, const double *__restrict__ fstate) {
-    const long long fid = (long long)blockIdx.x * kWavesT + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    __shared__ double tab[kFWaves][128];
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const long long fid = (long long)blockIdx.x * kFWaves + wv;
     if (fid >= nfaces) return;
     const int lane = threadIdx.x & 63;
     const int b = find_block(off, nb, fid);
@@ -347,66 +410,80 @@ __global__ void __launch_bounds__(kT) k_faceC(const FBlock *__restrict__ blocks,
     const Consts &C = B.c;
     const FaceLoc L = face_loc(B, fid - off[b]);
     const FaceGeo g = face_geo(B, L.gi, L.gj, L.face);
+    const Stencil<T> stn = stencil<T>(C, B.f, g);
     const double *sv = fstate + fid * kFT;
     const d4 prim = mk4(sv[0], sv[1], sv[2], sv[3]);
     const d4 aL = mk4(sv[4], sv[5], sv[6], sv[7]), aR = mk4(sv[8], sv[9], sv[10], sv[11]);
     const d4 aT = mk4(sv[12], sv[13], sv[14], sv[15]);
-    const TimeInt mt{sv[16], sv[17], sv[18], sv[19], sv[20]};
+    const double m0 = sv[16], m1 = sv[17], m2 = sv[18], m3 = sv[19], m4 = sv[20];
     const double Mxi2 = sv[21];
     const d2 Qf = mk2(sv[22], sv[23]);
     const double Lf = sv[28];
     const d2 n = mk2(sv[29], sv[30]);
-    // separable Maxwellian: lane i holds exp(-lam (a_i - c)^2) of axis 1, lane j of axis 2
+    // separable Maxwellian on a tensor grid with an axis-aligned normal:
+    // exp(-lam |uv - (U,V)|^2) = Ei[v / nvt] * Ej[v % nvt], one table pair per wave in LDS
     const bool sep = C.nut > 0 && (n.x == 0.0 || n.y == 0.0);
-    double Ei = 0.0, Ej = 0.0;
+    double *Ei = tab[wv], *Ej = tab[wv] + 64;
     if (sep) {
         const bool wtype = (n.y == 0.0);
         if (lane < C.nut) {
             const double a = wtype ? n.x * C.qu[lane] : -n.y * C.qu[lane], c = wtype ? prim.s1 : prim.s2;
-            Ei = exp(-prim.s3 * ((a - c) * (a - c)));
+            Ei[lane] = exp(-prim.s3 * ((a - c) * (a - c)));
         }
         if (lane < C.nvt) {
             const double a = wtype ? n.x * C.qv[lane] : n.y * C.qv[lane], c = wtype ? prim.s2 : prim.s1;
-            Ej = exp(-prim.s3 * ((a - c) * (a - c)));
+            Ej[lane] = exp(-prim.s3 * ((a - c) * (a - c)));
         }
     }
-    void *fbuf = L.face == SOUTH ? B.fluxS : B.fluxW;
+    // the tables are private to this wavefront: order its LDS writes before its reads
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    AS1 V2<T> *fout = (AS1 V2<T> *)(L.face == SOUTH ? B.fluxS : B.fluxW) + g.own * (size_t)C.NV;
     const double rl = prim.s0 * (prim.s3 / kPi);
     const double hk = C.K / (2.0 * prim.s3);
     const double sfac = 0.8 * (1 - C.Pr) * (prim.s3 * prim.s3) / prim.s0;
+    // per-velocity flux (distFlux, clUGKS.c:663-774) with the slope terms
+    // folded: for a = aL / aR / aT and P_a = a0 + a1 u + a2 w + a3 |u|^2 / 2,
+    //   the g part of the slope term is f0g (P_a + a3 hk / 2) and the h part
+    //   f0g (hk P_a + a3 Mxi2 / 2), with f0h = hk f0g
+    const double hL = 0.5 * aL.s3, hR = 0.5 * aR.s3, hT = 0.5 * aT.s3;
+    const double caL = hL * hk, caR = hR * hk, ct = hT * hk;
+    const double cmL = hL * Mxi2, cmR = hR * Mxi2, cmt = hT * Mxi2;
     double mf[4] = {0.0, 0.0, 0.0, 0.0};
-    const int nchunk = (C.NV + 63) >> 6;
-    for (int ck = 0; ck < nchunk; ck++) {
-        const int v = ck * 64 + lane;
-        const bool act = v < C.NV;
-        const int vv = act ? v : 0;
-        double E = 0.0;
-        if (sep) E = __shfl(Ei, vv / C.nvt, 64) * __shfl(Ej, vv % C.nvt, 64);
-        if (!act) continue;
+    // tensor indices of v = lane + 64 k, advanced without integer division
+    int iu = 0, jv = 0, diu = 0, djv = 0;
+    if (sep) { iu = lane / C.nvt; jv = lane - iu * C.nvt; diu = 64 / C.nvt; djv = 64 - diu * C.nvt; }
+    for (int v = lane; v < C.NV; v += 64) {
         const d2 uv = face_velocity(C, v, n);
-        P2<T> ff, sg, fl, fr;
-        reconstruct<T>(C, B.f, g, v, upwind(uv.x), ff, sg, fl, fr);
-        const double cx = uv.x - prim.s1, cy = uv.y - prim.s2, c2 = cx * cx + cy * cy;
-        if (!sep) E = exp(-prim.s3 * c2);
-        const double w = C.weight[v];
-        const double f0g = w * (rl * E), f0h = w * ((rl * E) * hk);
+        const int delta = upwind(uv.x);
+        P2<T> ff, sg;
+        reconstruct<T>(C, stn, g, v, delta, ff, sg);
+        const double ux = uv.x, uy = uv.y, cx = ux - prim.s1, cy = uy - prim.s2, c2 = cx * cx + cy * cy;
+        const double E = sep ? Ei[iu] * Ej[jv] : exp(-prim.s3 * c2);
+        if (sep) {
+            iu += diu; jv += djv;
+            if (jv >= C.nvt) { jv -= C.nvt; iu++; }
+        }
+        const double f0g = C.weight[v] * (rl * E);
         const double sh = sfac * (cx * Qf.x + cy * Qf.y), part = 2 * prim.s3 * c2;
-        const double F0g = sh * (part - 4) * f0g, F0h = sh * (part - 2) * f0h;
+        const double uu = ux * ux + uy * uy, ux2 = ux * ux;
+        const double PL = aL.s0 + aL.s1 * ux + aL.s2 * uy + hL * uu, PR = aR.s0 + aR.s1 * ux + aR.s2 * uy + hR * uu;
+        const double Pa = delta ? PL : PR, ca = delta ? caL : caR, cma = delta ? cmL : cmR;
+        const double Pt = aT.s0 + aT.s1 * ux + aT.s2 * uy + hT * uu;
         const double fdg = (double)ff.x, fdh = (double)ff.y, slg = (double)sg.x, slh = (double)sg.y;
-        const double ux = uv.x, uy = uv.y, uu = ux * ux + uy * uy, ux2 = ux * ux;
-        mf[0] += mt.m0 * ux * F0g + mt.m3 * ux * fdg - mt.m4 * ux2 * slg;
-        mf[1] += mt.m0 * ux2 * F0g + mt.m3 * ux2 * fdg - mt.m4 * ux2 * ux * slg;
-        mf[2] += mt.m0 * uy * ux * F0g + mt.m3 * uy * ux * fdg - mt.m4 * uy * ux2 * slg;
-        mf[3] += mt.m0 * 0.5 * (ux * uu * F0g + ux * F0h) + mt.m3 * 0.5 * (ux * uu * fdg + ux * fdh) -
-                 mt.m4 * 0.5 * (ux2 * uu * slg + ux2 * slh);
-        const d4 a = upwind(ux) ? aL : aR;
-        const double pg = aT.s0 * f0g + aT.s1 * ux * f0g + aT.s2 * uy * f0g + 0.5 * aT.s3 * (uu * f0g + f0h);
-        const double ph = aT.s0 * f0h + aT.s1 * ux * f0h + aT.s2 * uy * f0h + 0.5 * aT.s3 * (uu * f0h + Mxi2 * f0g);
-        const double sgv = a.s0 * f0g + a.s1 * ux * f0g + a.s2 * uy * f0g + 0.5 * a.s3 * (uu * f0g + f0h);
-        const double shv = a.s0 * f0h + a.s1 * ux * f0h + a.s2 * uy * f0h + 0.5 * a.s3 * (uu * f0h + Mxi2 * f0g);
-        const double rg = mt.m0 * ux * (f0g + F0g) + mt.m1 * ux2 * sgv + mt.m2 * ux * pg + mt.m3 * ux * fdg - mt.m4 * ux2 * slg;
-        const double rh = mt.m0 * ux * (f0h + F0h) + mt.m1 * ux2 * shv + mt.m2 * ux * ph + mt.m3 * ux * fdh - mt.m4 * ux2 * slh;
-        st<T>(fbuf, g.own * (size_t)C.NV + v, (T)(Lf * rg), (T)(Lf * rh));
+        const double tg = m3 * fdg - m4 * ux * slg, th = m3 * fdh - m4 * ux * slh;
+        const double shg = sh * (part - 4), shh = sh * (part - 2);
+        const double rg = f0g * (m0 * ux * (1.0 + shg) + m1 * ux2 * (Pa + ca) + m2 * ux * (Pt + ct)) + ux * tg;
+        const double rh = f0g * (m0 * ux * hk * (1.0 + shh) + m1 * ux2 * (hk * Pa + cma) + m2 * ux * (hk * Pt + cmt)) +
+                          ux * th;
+        fout[v] = V2<T>{(T)(Lf * rg), (T)(Lf * rh)};
+        // macroFlux (clUGKS.c:607-661): the Shakhov and the non-equilibrium parts
+        const double X = ux * (m0 * shg * f0g + tg);
+        mf[0] += X;
+        mf[1] += ux * X;
+        mf[2] += uy * X;
+        mf[3] += 0.5 * (uu * X + ux * (m0 * shh * (hk * f0g) + th));
     }
 #pragma unroll
     for (int q = 0; q < 4; q++) mf[q] = wave_sum(mf[q]);
@@ -453,14 +530,17 @@ __global__ void __launch_bounds__(kT) k_wall(const FBlock *__restrict__ blocks, 
     // reconstruction at the wall face (iFace) and the owner heat flux
     P2<T> ff[NPT];
     double acc[2] = {0.0, 0.0};
+    const Stencil<T> stn = stencil<T>(C, B.f, g);
 #pragma unroll
     for (int k = 0; k < NPT; k++) {
         const int v = threadIdx.x + kT * k;
         if (v < C.NV) {
             d2 uv = face_velocity(C, v, n);
-            P2<T> sgk, fl, fr;
-            reconstruct<T>(C, B.f, g, v, upwind(uv.x), ff[k], sgk, fl, fr);
+            P2<T> sgk;
+            reconstruct<T>(C, stn, g, v, upwind(uv.x), ff[k], sgk);
             if (wantQ) {
+                P2<T> fl, fr;
+                face_cells<T>(C, stn, v, fl, fr);
                 d2 e = C.quad[v];
                 double a = (double)fr.x, bq = (double)fr.y;
                 double cx = e.x - cm.s1, cy = e.y - cm.s2, t = (cx * cx + cy * cy) * a + bq / C.K;
@@ -689,9 +769,9 @@ __global__ void __launch_bounds__(kT) k_update(const FBlock *__restrict__ blocks
     const bool sep = C.nut > 0;
     if (sep) {
         for (int k = threadIdx.x; k < 2 * (C.nut + C.nvt); k += kT) {
-            int which = k / (C.nut + C.nvt), r = k % (C.nut + C.nvt);
+            const int which = k >= C.nut + C.nvt, r = which ? k - (C.nut + C.nvt) : k;
             const d4 &pp = which ? p : pold;
-            double a = r < C.nut ? C.qu[r] - pp.s1 : C.qv[r - C.nut] - pp.s2;
+            const double a = r < C.nut ? C.qu[r] - pp.s1 : C.qv[r - C.nut] - pp.s2;
             tab[2 * which + (r < C.nut ? 0 : 1)][r < C.nut ? r : r - C.nut] = exp(-pp.s3 * (a * a));
         }
         __syncthreads();
@@ -699,7 +779,17 @@ __global__ void __launch_bounds__(kT) k_update(const FBlock *__restrict__ blocks
     const double rlo = pold.s0 * (pold.s3 / kPi), rln = p.s0 * (p.s3 / kPi);
     const double so = 0.8 * (1 - C.Pr) * (pold.s3 * pold.s3) / pold.s0, sn = 0.8 * (1 - C.Pr) * (p.s3 * p.s3) / p.s0;
     const double hfo = C.K / (2.0 * pold.s3), hfn = C.K / (2.0 * p.s3);
-    const double rA = 1.0 / A, den = 1.0 + 0.5 * dt / tau;
+    const double rA = 1.0 / A, rden = 1.0 / (1.0 + 0.5 * dt / tau);
+    const double htn = (0.5 * dt) / tau, hto = (0.5 * dt) / tau_old;
+    // this cell's velocities in f and in the four face fluxes: uniform bases, 32-bit lane offsets
+    const long long NV = C.NV;
+    AS1 V2<T> *fc = (AS1 V2<T> *)B.f + c0 * NV;
+    const AS1 V2<T> *fS = (const AS1 V2<T> *)B.fluxS + c0 * NV;
+    const AS1 V2<T> *fW = (const AS1 V2<T> *)B.fluxW + c0 * NV;
+    const unsigned nS = (unsigned)NV, nE = (unsigned)(C.NJ * NV);
+    // tensor indices of v = threadIdx.x + 256 k, advanced without integer division
+    int iu = 0, jv = 0, diu = 0, djv = 0;
+    if (sep) { iu = threadIdx.x / C.nvt; jv = threadIdx.x - iu * C.nvt; diu = kT / C.nvt; djv = kT - diu * C.nvt; }
     bool nan_seen = false;
 #pragma unroll
     for (int k = 0; k < NPT; k++) {
@@ -707,35 +797,35 @@ __global__ void __launch_bounds__(kT) k_update(const FBlock *__restrict__ blocks
         if (v < C.NV) {
             const d2 e = C.quad[v];
             const double w = C.weight[v];
+            const double ox = e.x - pold.s1, oy = e.y - pold.s2, nx = e.x - p.s1, ny = e.y - p.s2;
+            const double co = ox * ox + oy * oy, cn = nx * nx + ny * ny;
             double Eo, En;
             if (sep) {
-                int iu = v / C.nvt, jv = v % C.nvt;
                 Eo = tab[0][iu] * tab[1][jv];
                 En = tab[2][iu] * tab[3][jv];
             } else {
-                double ox = e.x - pold.s1, oy = e.y - pold.s2, nx = e.x - p.s1, ny = e.y - p.s2;
-                Eo = exp(-pold.s3 * (ox * ox + oy * oy));
-                En = exp(-p.s3 * (nx * nx + ny * ny));
+                Eo = exp(-pold.s3 * co);
+                En = exp(-p.s3 * cn);
             }
-            double ox = e.x - pold.s1, oy = e.y - pold.s2, nx = e.x - p.s1, ny = e.y - p.s2;
-            double gO = w * (rlo * Eo), hO = gO * hfo;
-            double shO = so * (ox * Q.x + oy * Q.y), ptO = 2 * pold.s3 * (ox * ox + oy * oy);
-            double feqOg = gO + shO * (ptO - 4) * gO, feqOh = hO + shO * (ptO - 2) * hO;
-            double gN = w * (rln * En), hN = gN * hfn;
-            double shN = sn * (nx * Q.x + ny * Q.y), ptN = 2 * p.s3 * (nx * nx + ny * ny);
-            double feqNg = gN + shN * (ptN - 4) * gN, feqNh = hN + shN * (ptN - 2) * hN;
-            const size_t kk = c0 * (size_t)C.NV + v;
-            P2<T> fo = ld<T>(B.f, kk);
-            P2<T> sS = ld<T>(B.fluxS, kk), nS = ld<T>(B.fluxS, kk + C.NV);
-            P2<T> wW = ld<T>(B.fluxW, kk), eW = ld<T>(B.fluxW, kk + (size_t)C.NJ * C.NV);
-            double flg = (((double)sS.x - (double)nS.x) + (double)wW.x - (double)eW.x) * rA;
-            double flh = (((double)sS.y - (double)nS.y) + (double)wW.y - (double)eW.y) * rA;
-            double fg = (double)fo.x, fh = (double)fo.y;
-            double ng = (fg + flg + (0.5 * dt) * (feqNg / tau + (feqOg - fg) / tau_old)) / den;
-            double nh = (fh + flh + (0.5 * dt) * (feqNh / tau + (feqOh - fh) / tau_old)) / den;
-            T tg = (T)ng, th = (T)nh;
+            const double gO = w * (rlo * Eo), hO = gO * hfo;
+            const double shO = so * (ox * Q.x + oy * Q.y), ptO = 2 * pold.s3 * co;
+            const double feqOg = gO + shO * (ptO - 4) * gO, feqOh = hO + shO * (ptO - 2) * hO;
+            const double gN = w * (rln * En), hN = gN * hfn;
+            const double shN = sn * (nx * Q.x + ny * Q.y), ptN = 2 * p.s3 * cn;
+            const double feqNg = gN + shN * (ptN - 4) * gN, feqNh = hN + shN * (ptN - 2) * hN;
+            const V2<T> fo = fc[v], sS = fS[v], nS_ = fS[v + nS], wW = fW[v], eW = fW[v + nE];
+            const double flg = (((double)sS.x - (double)nS_.x) + (double)wW.x - (double)eW.x) * rA;
+            const double flh = (((double)sS.y - (double)nS_.y) + (double)wW.y - (double)eW.y) * rA;
+            const double fg = (double)fo.x, fh = (double)fo.y;
+            const double ng = (fg + flg + (htn * feqNg + hto * (feqOg - fg))) * rden;
+            const double nh = (fh + flh + (htn * feqNh + hto * (feqOh - fh))) * rden;
+            const T tg = (T)ng, th = (T)nh;
             if (isnan(tg) || isnan(th)) nan_seen = true;
-            st<T>(B.f, kk, tg, th);
+            fc[v] = V2<T>{tg, th};
+        }
+        if (sep) {
+            iu += diu; jv += djv;
+            if (jv >= C.nvt) { jv -= C.nvt; iu++; }
         }
     }
     if (nan_seen) B.flag[0] = ERR_NAN;
@@ -986,11 +1076,11 @@ namespace {
 // ev != nullptr: record ev[0..kLaunches] around the launches (per-kernel timing)
 constexpr int kLaunches = UGKS_STEP_NKERNELS;
 template <typename T, int NPT> int launch_step(ugks_step_plan *p, hipStream_t s, hipEvent_t *ev) {
-    const unsigned gw = (unsigned)((p->nfaces + kWavesT - 1) / kWavesT);   // one wavefront per face
+    const unsigned gw = (unsigned)((p->nfaces + kFWaves - 1) / kFWaves);   // one wavefront per face
     const unsigned gt = (unsigned)((p->nfaces + kT - 1) / kT);             // one thread per face
     if (ev) (void)hipEventRecord(ev[0], s);
     if (p->nfaces > 0)
-        hipLaunchKernelGGL((k_faceA<T>), dim3(gw), dim3(kT), 0, s, p->d_blocks, p->d_face_off, p->nb, p->nfaces,
+        hipLaunchKernelGGL((k_faceA<T>), dim3(gw), dim3(kFW), 0, s, p->d_blocks, p->d_face_off, p->nb, p->nfaces,
                            p->d_fsum);
     if (ev) (void)hipEventRecord(ev[1], s);
     if (p->nfaces > 0)
@@ -998,7 +1088,7 @@ template <typename T, int NPT> int launch_step(ugks_step_plan *p, hipStream_t s,
                            (const double *)p->d_fsum, p->d_fstate, (const DtState *)p->d_state);
     if (ev) (void)hipEventRecord(ev[2], s);
     if (p->nfaces > 0)
-        hipLaunchKernelGGL((k_faceC<T>), dim3(gw), dim3(kT), 0, s, p->d_blocks, p->d_face_off, p->nb, p->nfaces,
+        hipLaunchKernelGGL((k_faceC<T>), dim3(gw), dim3(kFW), 0, s, p->d_blocks, p->d_face_off, p->nb, p->nfaces,
                            (const double *)p->d_fstate);
     if (ev) (void)hipEventRecord(ev[3], s);
     if (p->nwalls > 0)
