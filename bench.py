@@ -1,21 +1,28 @@
 #!/usr/bin/env python3
 """Benchmark of the UGKS time step on one MI355X.
 
-Workload (default): the lid-driven cavity split into 2x2 blocks of 128x128
+Default workload: the lid-driven cavity split into 2x2 blocks of 128x128
 cells (256x256 in total) with ghost exchange between the blocks, 33x33
 Newton-Cotes velocities on [-6,6]^2 (the "32x32" grid after the reference's
 n -> floor(n/4)*4+1 rounding), WENO5, Kn = 0.01, lid 0.15, CFL 0.8, FP64.
 Synthetic input: the initial state is the Maxwellian at rho = 1, U = 0,
-T = 1 (no data files).
+T = 1 (no data files).  Other workloads (--workload): cavity_64 (64^2,
+Gauss-Hermite 24^2, vanLeer, Kn 0.075), cavity_512 (512^2, Newton-Cotes
+49^2, rho perturbed by 1%, --kn), batch64 (64 seeded cavities of uneven
+size in one plan), channel (adsorbing-wall channel, 8 blocks, 1024x256,
+41^2 velocities).
 
-A step is one full time step of all four blocks (flux, update, time step,
-ghost exchange) through the fused FP64 path; the metric is
-cell-velocity updates per second = sum(ni*nj*Nv) * steps / seconds, with
-all state resident in HBM.  The same workload is also run with the mixed
-FP32/FP64 step and its error against the FP64 step is reported, and the
-FP64 CPU oracle is timed on a bounded sample on the host cores.
+A step is one full time step of every block of every case (flux, update,
+time step, ghost exchange) through the fused path, replayed as a HIP graph
+(--no-graph: eager launches); the metric is cell-velocity updates per
+second = sum(ni*nj*Nv) * steps / seconds, with all state resident in HBM.
+The same workload is also run with the mixed FP32/FP64 step and its error
+against the FP64 step is reported (L-inf and L2 of rho, U, T, q and the
+mass drift of both), and the FP64 CPU oracle is timed on a bounded sample
+on the host cores, with the FP64 GPU state compared to it after the same
+steps.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
 """
 import argparse
 import copy
@@ -29,6 +36,8 @@ sys.path.insert(0, ROOT)
 
 METRIC = "cell·velocity updates/s + % HBM roofline, mixed vs FP64 at ≤1e-5 rel err"
 HBM_PEAK_GBS = 8000.0
+CUS, CLOCK_HZ = 256, 2.4e9      # MI355X: one full-rate wave64 VALU instruction per CU per clock
+WORKLOADS = ("cavity_2x2", "cavity_64", "cavity_512", "batch64", "channel")
 
 
 def parse():
@@ -36,9 +45,12 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--cells", type=int, default=128, help="cells per block side (2x2 blocks)")
-    ap.add_argument("--nc", type=int, default=32, help="Newton-Cotes points per axis before rounding")
-    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--workload", choices=WORKLOADS, default="cavity_2x2")
+    ap.add_argument("--cells", type=int, default=128, help="cavity_2x2: cells per block side")
+    ap.add_argument("--nc", type=int, default=32, help="cavity_2x2: Newton-Cotes points per axis before rounding")
+    ap.add_argument("--kn", type=float, default=0.075, help="cavity_512: Knudsen number")
+    ap.add_argument("--precision", choices=("both", "fp64", "mixed"), default="both")
+    ap.add_argument("--no-graph", action="store_true", help="time eager launches instead of graph replays")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--profile-steps", type=int, default=5, help="steps timed per kernel with HIP events")
@@ -74,10 +86,21 @@ def host_cores():
     return max(1, min(n, 16))
 
 
-def read_traffic(kernel, workload):
-    """HBM bytes per launch measured by rocprofv3 (FETCH_SIZE + WRITE_SIZE),
-    from the summary committed under profiles/ for this kernel and workload."""
-    path = os.path.join(ROOT, "profiles", "traffic.txt")
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def read_profile_table(fname, kernel, workload):
+    """A per-launch figure for (kernel, workload) from a summary committed
+    under profiles/ (lines "kernel workload value ...")."""
+    path = os.path.join(ROOT, "profiles", fname)
     if not os.path.exists(path):
         return None
     with open(path) as fh:
@@ -89,6 +112,19 @@ def read_traffic(kernel, workload):
                 except ValueError:
                     return None
     return None
+
+
+def build_cases(a):
+    from pyugks_amd import cases
+    if a.workload == "cavity_2x2":
+        return [cases.cfg2_cavity_2x2(n_per_block=a.cells, nc=a.nc)], f"cavity_2x2x{a.cells}sq_nc{a.nc}_weno5"
+    if a.workload == "cavity_64":
+        return [cases.cfg1_cavity_gh()], "cavity_64sq_gh24_vanleer"
+    if a.workload == "cavity_512":
+        return [cases.cfg3_cavity_512(Kn=a.kn)], f"cavity_512sq_nc48_weno5_kn{a.kn:g}"
+    if a.workload == "batch64":
+        return cases.cfg4_batch(), "batch64_cavities_weno5"
+    return [cases.cfg5_channel()], "channel_adsorbing_1024x256_nc40_weno5"
 
 
 def timed_run(plan, steps, warmup):
@@ -105,16 +141,97 @@ def timed_run(plan, steps, warmup):
     return t1 - t0, e0.elapsed_ms(e1) / 1e3
 
 
+def run_precision(gs, precision, a, upd_per_step, workload):
+    """Time one precision; returns its report and the final per-case state."""
+    from pyugks_amd.fused import StepPlan
+    from pyugks_amd.sim import UGKSim
+    sims = [UGKSim(copy.deepcopy(g)) for g in gs]
+    mass0 = [sum(s.total_mass()) for s in sims]
+    plan = StepPlan(sims, precision, graph=not a.no_graph)
+    print(f"[bench] {precision}: {len(plan.blocks)} blocks in {len(sims)} case(s)", file=sys.stderr, flush=True)
+    wall_s, ev_s = timed_run(plan, a.steps, a.warmup)
+    value = upd_per_step * a.steps / wall_s
+    print(f"[bench] {precision}: {a.steps} steps in {wall_s:.4f} s ({ev_s:.4f} s by events)",
+          file=sys.stderr, flush=True)
+    kern = plan.run_timed(a.profile_steps)
+    per_launch = {k: v / a.profile_steps for k, v in kern.items()}
+    dom = max(per_launch, key=per_launch.get)
+    b_alg = 32.0 if precision == "fp64" else 16.0
+    achieved = b_alg * upd_per_step / (per_launch[dom] * 1e-3) / 1e9
+    tag = f"{workload}_{precision}"
+    roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": read_profile_table("traffic.txt", dom, tag),
+            "kernel": dom, "kernel_ms": per_launch[dom], "b_alg_per_update": b_alg,
+            "step_achieved_gbs": value * b_alg / 1e9}
+    valu = read_profile_table("valu.txt", dom, tag)
+    if valu is not None:
+        roof["valu_insts"] = valu
+        roof["valu_frac"] = valu / (per_launch[dom] * 1e-3 * CUS * CLOCK_HZ)
+    states = plan.states()
+    final = [{"macro": [b.macro() for b in s.blocks], "q": [b.Q_D.download() for b in s.blocks],
+              "mass": sum(s.total_mass())} for s in sims]
+    rep = {"value": value, "ms_per_step": 1e3 * wall_s / a.steps, "roofline": roof,
+           "kernel_ms_per_launch": per_launch, "flag": plan.flag(),
+           "steps_done": [d.step for d in states],
+           "mass_drift": max(abs(f["mass"] - m0) / m0 for f, m0 in zip(final, mass0)),
+           "graph": not a.no_graph}
+    plan.close()
+    for s in sims:
+        for b in s.blocks:
+            b.free_device()
+    return rep, final
+
+
+def errors_between(final_a, final_b):
+    from pyugks_amd.metrics import field_errors
+    worst = {}
+    for fa, fb in zip(final_a, final_b):
+        e = field_errors(fa["macro"], fb["macro"], fa["q"], fb["q"])
+        for k, v in e.items():
+            worst[k] = max(worst.get(k, 0.0), v)
+    return worst
+
+
+def cpu_baseline(gs, a, upd_per_step):
+    """The FP64 oracle on the host cores over a bounded number of steps, and
+    the FP64 GPU step over the same steps from the same initial state."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    from pyugks_amd.fused import StepPlan
+    from pyugks_amd.sim import UGKSim
+    cores = host_cores()
+    print(f"[bench] cpu baseline on {cores} threads", file=sys.stderr, flush=True)
+    oras = [O.OracleSim(copy.deepcopy(g), threads=cores) for g in gs]
+    t0 = time.perf_counter()
+    for o in oras:
+        o.run(1)
+    one = time.perf_counter() - t0
+    n = max(1, int(a.cpu_seconds / max(one, 1e-3)) - 1)
+    t0 = time.perf_counter()
+    for o in oras:
+        o.run(n)
+    dt = time.perf_counter() - t0
+    out = {"value": upd_per_step * n / dt, "unit": "cell-velocity updates/s", "cores": cores, "kind": "port",
+           "sample": f"{n} steps of the same workload after 1 untimed step, FP64 C/OpenMP oracle, {dt:.1f} s "
+                     f"on {cores} threads of {cpu_model()}"}
+    sims = [UGKSim(copy.deepcopy(g)) for g in gs]
+    plan = StepPlan(sims, "fp64")
+    plan.run(n + 1)
+    final_gpu = [{"macro": [b.macro() for b in s.blocks], "q": [b.Q_D.download() for b in s.blocks]} for s in sims]
+    final_ora = [{"macro": [b["macro"] for b in o.blocks], "q": [b["Q"] for b in o.blocks]} for o in oras]
+    out["fp64_gpu_vs_oracle"] = errors_between(final_gpu, final_ora)
+    out["fp64_gpu_vs_oracle"]["steps"] = n + 1
+    plan.close()
+    return out
+
+
 def main():
     a = parse()
     if a.gpus != 1:
         print(f"bench.py: runs on one GPU only; --gpus {a.gpus} is not supported (blocks would be "
               "sharded one group per GPU with RCCL ghost exchange, which is not built)", file=sys.stderr)
         sys.exit(2)
-    from pyugks_amd import cases
     from pyugks_amd import lib as L
-    from pyugks_amd.fused import StepPlan
-    from pyugks_amd.sim import UGKSim
     L.load()
     if L.device_count() < 1:
         print("bench.py: no GPU visible", file=sys.stderr)
@@ -123,90 +240,53 @@ def main():
         calibrate_copy(a.calibrate_copy)
         return
 
-    g0 = cases.cfg2_cavity_2x2(n_per_block=a.cells, nc=a.nc)
-    workload = f"cavity_2x2x{a.cells}sq_nc{a.nc}_weno5"
+    gs, workload = build_cases(a)
     print(f"[bench] init {workload}", file=sys.stderr, flush=True)
-    sim = UGKSim(copy.deepcopy(g0))
-    nv = sim.gdata.Nv
-    upd_per_step = sum(b.ni * b.nj for b in sim.blocks) * nv
-    plan = StepPlan(sim, "fp64")
-    wall_s, ev_s = timed_run(plan, a.steps, a.warmup)
-    value = upd_per_step * a.steps / wall_s
-    print(f"[bench] fp64 {a.steps} steps in {wall_s:.4f} s ({ev_s:.4f} s by events)", file=sys.stderr, flush=True)
-    kern = plan.run_timed(a.profile_steps)
-    per_launch = {k: v / a.profile_steps for k, v in kern.items()}
-    dom = max(per_launch, key=per_launch.get)
-    b_alg = 32.0  # (1 read + 1 write) x (g, h) x 8 B per cell-velocity update
-    achieved = b_alg * upd_per_step / (per_launch[dom] * 1e-3) / 1e9
-    traffic = read_traffic(dom, workload)
-    macro64 = [b.macro() for b in sim.blocks]
-    steps64 = plan.state().step
-    flag64 = plan.flag()
-
+    upd_per_step = 0
+    nv_set = set()
+    for g in gs:
+        gp = copy.deepcopy(g).prepare()
+        for b in gp.blocks:
+            upd_per_step += b.ni * b.nj * gp.Nv
+        nv_set.add(gp.Nv)
+    prec = ["fp64", "mixed"] if a.precision == "both" else [a.precision]
+    reps, finals = {}, {}
+    for p in prec:
+        reps[p], finals[p] = run_precision(gs, p, a, upd_per_step, workload)
+    head = prec[0]
+    r = reps[head]
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": r["value"],
         "unit": "cell-velocity updates/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": 1e3 * wall_s / a.steps,
+        "ms_per_step": r["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic (Maxwellian initial state, no input files)",
-        "config": {"workload": workload, "blocks": len(sim.blocks), "cells": upd_per_step // nv, "Nv": nv,
-                   "reconstruction": "WENO5", "Kn": 0.01, "lid": 0.15, "CFL": 0.8, "path": "fused fp64"},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom,
-                     "kernel_ms": per_launch[dom], "b_alg_per_update": b_alg,
-                     "step_achieved_gbs": value * b_alg / 1e9},
-        "kernel_ms_per_launch": per_launch,
-        "flag": flag64,
+        "dtype": "f64" if head == "fp64" else "f32 storage/flux, f64 moments",
+        "data": "synthetic (Maxwellian initial state, seeded perturbations; no input files)",
+        "config": {"workload": workload, "cases": len(gs), "updates_per_step": upd_per_step,
+                   "Nv": sorted(nv_set), "path": f"fused {head}" + (" graph" if r["graph"] else "")},
+        "roofline": r["roofline"],
+        "kernel_ms_per_launch": r["kernel_ms_per_launch"],
+        "mass_drift": r["mass_drift"],
+        "flag": r["flag"],
     }
-
-    if not a.no_mixed:
-        from pyugks_amd.metrics import field_errors
-        simm = UGKSim(copy.deepcopy(g0))
-        planm = StepPlan(simm, "mixed")
-        wall_m, _ = timed_run(planm, a.steps, a.warmup)
-        vm = upd_per_step * a.steps / wall_m
-        km = planm.run_timed(a.profile_steps)
-        plm = {k: v / a.profile_steps for k, v in km.items()}
-        domm = max(plm, key=plm.get)
-        # compare after the same number of steps as the FP64 state above
-        extra = steps64 - planm.state().step
-        if extra > 0:
-            planm.run(extra)
-        errs = field_errors([b.macro() for b in simm.blocks], macro64)
-        ach_m = 16.0 * upd_per_step / (plm[domm] * 1e-3) / 1e9
-        out["mixed"] = {"value": vm, "ms_per_step": 1e3 * wall_m / a.steps, "dtype": "f32 storage/flux, f64 moments",
-                        "speedup_vs_fp64": vm / value,
-                        "roofline": {"bound": "hbm", "achieved": ach_m, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                                     "frac": ach_m / HBM_PEAK_GBS, "kernel": domm, "kernel_ms": plm[domm],
-                                     "b_alg_per_update": 16.0, "step_achieved_gbs": vm * 16.0 / 1e9},
-                        "err_vs_fp64": errs, "kernel_ms_per_launch": plm, "flag": planm.flag()}
-        del planm, simm
-
+    if "fp64" in reps and "mixed" in reps:
+        m = reps["mixed"]
+        assert m["steps_done"] == reps["fp64"]["steps_done"]
+        out["mixed"] = {"value": m["value"], "ms_per_step": m["ms_per_step"],
+                        "dtype": "f32 storage/flux, f64 moments", "speedup_vs_fp64": m["value"] / r["value"],
+                        "roofline": m["roofline"], "err_vs_fp64": errors_between(finals["mixed"], finals["fp64"]),
+                        "mass_drift": m["mass_drift"], "kernel_ms_per_launch": m["kernel_ms_per_launch"],
+                        "flag": m["flag"]}
+    finals.clear()
     if not a.no_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import oracle as O
-        cores = host_cores()
-        print(f"[bench] cpu baseline on {cores} threads", file=sys.stderr, flush=True)
-        ora = O.OracleSim(copy.deepcopy(g0), threads=cores)
-        t0 = time.perf_counter()
-        ora.run(1)
-        one = time.perf_counter() - t0
-        n = max(1, int(a.cpu_seconds / max(one, 1e-3)) - 1)
-        t0 = time.perf_counter()
-        ora.run(n)
-        dt = time.perf_counter() - t0
-        out["cpu_baseline"] = {"value": upd_per_step * n / dt, "unit": "cell-velocity updates/s", "cores": cores,
-                               "kind": "port",
-                               "sample": f"{n} steps of the same workload (after 1 calibration step), FP64 "
-                                         f"C/OpenMP oracle, {dt:.1f} s"}
-        out["cpu_baseline"]["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
+        out["cpu_baseline"] = cpu_baseline(gs, a, upd_per_step)
+        out["cpu_baseline"]["gpu_over_cpu"] = r["value"] / out["cpu_baseline"]["value"]
     print(json.dumps(out), flush=True)
 
 

@@ -223,15 +223,29 @@ typedef struct ugks_step_plan ugks_step_plan;
 
 int ugks_step_plan_create(ugks_block *const *blocks, const ugks_block_buffers *bufs, const ugks_bc_desc *bcs,
                           int32_t nblocks, int32_t precision, const ugks_dt_desc *dt, ugks_step_plan **out);
+/* Many independent cases in one plan: block b belongs to case case_of_block[b]
+ * (blocks of a case contiguous, cases numbered 0..ncases-1 in block order);
+ * dt[ncases] holds each case's time-step state, which evolves separately as in
+ * one UGKSim per case (ugks_sim.py:285-311).  bcs[].other_block indexes the
+ * plan's block list and must stay within the block's case. */
+int ugks_step_plan_create_batch(ugks_block *const *blocks, const ugks_block_buffers *bufs, const ugks_bc_desc *bcs,
+                                int32_t nblocks, const int32_t *case_of_block, int32_t ncases, int32_t precision,
+                                const ugks_dt_desc *dt, ugks_step_plan **out);
 int ugks_step_plan_destroy(ugks_step_plan *plan);
 /* nsteps full steps: flux, update, time step, BC refresh (ugks_sim.py:407-501) */
 int ugks_step_run(ugks_step_plan *plan, void *stream, int32_t nsteps);
+/* the same steps as replays of a HIP graph of one step, captured on first use
+ * (the step has no host round trip, so the graph is valid for the plan's life) */
+int ugks_step_run_graph(ugks_step_plan *plan, void *stream, int32_t nsteps);
 /* the same steps with HIP events around each launch; kernel_ms[UGKS_STEP_NKERNELS]
  * receives the summed time of each launch of the step, in the order above */
 int ugks_step_run_timed(ugks_step_plan *plan, void *stream, int32_t nsteps, float *kernel_ms);
-/* read back dt, time, step (and the per-block maximum time steps if max_dt != NULL) */
+/* read back dt, time, step of case 0 (and the per-block maximum time steps if max_dt != NULL) */
 int ugks_step_state(ugks_step_plan *plan, void *stream, ugks_dt_desc *out, double *max_dt);
+/* the same for every case: out[ncases], max_dt[nblocks] (may be NULL) */
+int ugks_step_states(ugks_step_plan *plan, void *stream, ugks_dt_desc *out, double *max_dt);
 int ugks_step_set_dt(ugks_step_plan *plan, void *stream, double dt);
+int ugks_step_set_dt_case(ugks_step_plan *plan, void *stream, int32_t case_index, double dt);
 /* plain device-to-device copy kernel of `bytes` (a multiple of 16): the
  * reference stream for bandwidth and byte-counter calibration */
 int ugks_copy_kernel(void *stream, const void *src, void *dst, int64_t bytes);

@@ -253,6 +253,12 @@ class UGKSBlock:
         if g.suggest_dt:
             self.max_dt = min(g.suggest_dt, self.max_dt)
 
+    def free_device(self):
+        """Release every device buffer of this block now (instead of at garbage collection)."""
+        for k, v in list(vars(self).items()):
+            if isinstance(v, L.DeviceArray):
+                v.free()
+
     # ------------------------------------------------------------- host data
     def macro(self):
         return self.macro_D.download(self.stream)

@@ -76,6 +76,7 @@ struct FBlock {
     int bc[4], other[4], oface[4], ori[4], flip[4], reflect[4];
     int nS, nSj, ob_S, ob_W, nWi, wl;       // interior-face counts and offsets
     int flipidx[4];                         // exchange row flip from the corner check
+    int cs;                                 // case of this block (its time-step state)
 };
 
 struct DtState {
@@ -373,7 +374,7 @@ __global__ void __launch_bounds__(kT) k_faceS(const FBlock *__restrict__ blocks,
     velocity_moments(C, prim, m);
     const d4 sw = (-prim.s0) * (moment_au(aL, m.MuL, m.Mv, m.Mxi, 1, 0) + moment_au(aR, m.MuR, m.Mv, m.Mxi, 1, 0));
     const d4 aT = micro_slope(C, prim, sw);
-    const TimeInt mt = time_integrals(relax_time(C, prim), ds->dt);
+    const TimeInt mt = time_integrals(relax_time(C, prim), ds[B.cs].dt);
     const d4 g0 = prim.s0 * (mt.m0 * moment_uv(m.Mu, m.Mv, m.Mxi, 1, 0, 0) +
                              mt.m1 * (moment_au(aL, m.MuL, m.Mv, m.Mxi, 2, 0) + moment_au(aR, m.MuR, m.Mv, m.Mxi, 2, 0)) +
                              mt.m2 * moment_au(aT, m.Mu, m.Mv, m.Mxi, 1, 0));
@@ -519,7 +520,7 @@ __global__ void __launch_bounds__(kT) k_wall(const FBlock *__restrict__ blocks, 
     case GSOUTH: gi = ci + C.G; gj = C.G; rot = 1; fid = SOUTH; break;
     default: gi = C.G; gj = ci + C.G; rot = 1; fid = WEST; break;
     }
-    const double dt = ds->dt;
+    const double dt = ds[B.cs].dt;
     const FaceGeo g = face_geo(B, gi, gj, fid);
     const d2 n = B.normal[g.own * 2 + fid];
     const double L = B.side_length[g.own * 2 + fid];
@@ -753,7 +754,7 @@ __global__ void __launch_bounds__(kT) k_update(const FBlock *__restrict__ blocks
     const long long cell = item - off[b];
     const int mi = (int)(cell / C.nj), mj = (int)(cell % C.nj), gi = mi + C.G, gj = mj + C.G;
     const size_t c0 = (size_t)gi * C.NJ + gj;
-    const double dt = ds->dt;
+    const double dt = ds[B.cs].dt;
     // updateMacro (clUGKS.c:1158-1187)
     const d4 pold = B.macro[cell];
     const double A = B.area[c0];
@@ -858,7 +859,7 @@ __global__ void __launch_bounds__(kT) k_dt(const FBlock *__restrict__ blocks, co
         double sos = sqrt(0.5 * C.gam / m.s3);
         double lo_u = fmin(C.umax, fabs(m.s1) + sos), lo_v = fmin(C.vmax, fabs(m.s2) + sos);
         double hi_u = fmax(C.umax, fabs(m.s1) + sos), hi_v = fmax(C.vmax, fabs(m.s2) + sos);
-        double scale = (erf(10.0 * (ds->Kn - 1e-3)) + 1) / 2.0;
+        double scale = (erf(10.0 * (ds[B.cs].Kn - 1e-3)) + 1) / 2.0;
         double u = scale * (hi_u - lo_u) + lo_u, v = scale * (hi_v - lo_v) + lo_v;
         val = (dx * u + dy * v) / B.area[(size_t)(mi + G) * C.NJ + mj + G];
         B.time_step[cell] = val;
@@ -882,13 +883,16 @@ __global__ void __launch_bounds__(kT) k_dt(const FBlock *__restrict__ blocks, co
     }
 }
 
-// the global time step (UGKSBlock.getDT + UGKSim.updateTimeStep), applied
-// once at the end of the step and once more when the next step's flux loop
-// re-evaluates it (ugks_sim.py:430-441); time advances by the first value
-__global__ void k_dtfin(DtState *ds, unsigned long long *maxbits, double *max_dt, int nb) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// the global time step of each case (UGKSBlock.getDT + UGKSim.updateTimeStep),
+// applied once at the end of the step and once more when the next step's flux
+// loop re-evaluates it (ugks_sim.py:430-441); time advances by the first
+// value.  One thread per case; the blocks of case c are first[c] .. first[c+1]-1.
+__global__ void k_dtfin(DtState *dss, unsigned long long *maxbits, double *max_dt, const int *first, int ncases) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncases) return;
+    DtState *ds = dss + c;
     double M = __builtin_inf();
-    for (int b = 0; b < nb; b++) {
+    for (int b = first[c]; b < first[c + 1]; b++) {
         double mx = __longlong_as_double((long long)maxbits[b]);
         double md = ds->CFL / mx;
         if (ds->suggest_dt > 0.0 && ds->suggest_dt < md) md = ds->suggest_dt;
@@ -1065,8 +1069,13 @@ struct ugks_step_plan {
     GhostItem *d_ghosts;
     int nwall_items, nghost_items;
     long long nfaces, nwalls, ncells, nghosts;
-    DtState *d_state;
+    int ncases;
+    DtState *d_state;                   // [ncases]
+    int *d_case_first;                  // [ncases + 1] first block of each case
     double *d_fsum, *d_fstate;          // per interior face: raw sums, face state
+    hipStream_t cap_stream;             // stream the step graph is captured on
+    hipGraph_t graph;
+    hipGraphExec_t graph_exec;
     unsigned long long *d_maxbits;
     double *d_max_dt;
     std::vector<FBlock> host_blocks;
@@ -1101,7 +1110,8 @@ template <typename T, int NPT> int launch_step(ugks_step_plan *p, hipStream_t s,
     hipLaunchKernelGGL(k_dt, dim3((unsigned)((p->ncells + kT - 1) / kT)), dim3(kT), 0, s, p->d_blocks, p->d_cell_off,
                        p->nb, p->d_state, p->d_maxbits);
     if (ev) (void)hipEventRecord(ev[6], s);
-    hipLaunchKernelGGL(k_dtfin, dim3(1), dim3(64), 0, s, p->d_state, p->d_maxbits, p->d_max_dt, p->nb);
+    hipLaunchKernelGGL(k_dtfin, dim3((unsigned)((p->ncases + 63) / 64)), dim3(64), 0, s, p->d_state, p->d_maxbits,
+                       p->d_max_dt, (const int *)p->d_case_first, p->ncases);
     if (ev) (void)hipEventRecord(ev[7], s);
     if (p->nghosts > 0)
         hipLaunchKernelGGL((k_ghost<T>), dim3((unsigned)p->nghosts), dim3(kT), 0, s, p->d_blocks, p->d_ghosts,
@@ -1127,13 +1137,30 @@ template <typename T> int dispatch_npt(ugks_step_plan *p, hipStream_t s, hipEven
 
 extern "C" {
 
-int ugks_step_plan_create(ugks_block *const *blocks, const ugks_block_buffers *bufs, const ugks_bc_desc *bcs,
-                          int32_t nblocks, int32_t precision, const ugks_dt_desc *dtd, ugks_step_plan **out) {
-    if (!blocks || !bufs || !bcs || !dtd || !out || nblocks < 1) return ugks_rt::fail(-1, "step plan: bad arguments");
+int ugks_step_plan_create_batch(ugks_block *const *blocks, const ugks_block_buffers *bufs, const ugks_bc_desc *bcs,
+                                int32_t nblocks, const int32_t *case_of_block, int32_t ncases, int32_t precision,
+                                const ugks_dt_desc *dtd, ugks_step_plan **out) {
+    if (!blocks || !bufs || !bcs || !dtd || !out || nblocks < 1 || ncases < 1)
+        return ugks_rt::fail(-1, "step plan: bad arguments");
     if (precision != UGKS_FP64 && precision != UGKS_MIXED) return ugks_rt::fail(-1, "step plan: bad precision");
     *out = nullptr;
+    // blocks of one case are contiguous and cases are numbered in block order
+    std::vector<int> case_first(ncases + 1, 0);
+    {
+        int prev = 0;
+        for (int b = 0; b < nblocks; b++) {
+            const int c = case_of_block ? case_of_block[b] : 0;
+            if (c < 0 || c >= ncases || c < prev || c > prev + 1 || (b == 0 && c != 0))
+                return ugks_rt::fail(-1, "step plan: case_of_block must run 0,0,..,1,1,.. up to ncases-1");
+            if (c != prev) case_first[c] = b;
+            prev = c;
+        }
+        if (prev != ncases - 1) return ugks_rt::fail(-1, "step plan: a case has no blocks");
+        case_first[ncases] = nblocks;
+    }
     ugks_step_plan *p = new ugks_step_plan();
     p->nb = nblocks;
+    p->ncases = ncases;
     p->precision = precision;
     int maxnv = 0;
     std::vector<long long> face_off(nblocks + 1, 0), cell_off(nblocks + 1, 0);
@@ -1148,6 +1175,7 @@ int ugks_step_plan_create(ugks_block *const *blocks, const ugks_block_buffers *b
         FBlock &F = p->host_blocks[b];
         memset(&F, 0, sizeof F);
         F.c = C;
+        F.cs = case_of_block ? case_of_block[b] : 0;
         F.f = u.f; F.fluxS = u.flux_f_S; F.fluxW = u.flux_f_W; F.wdist = u.wall_dist;
         F.fmS = (d4 *)u.flux_macro_S; F.fmW = (d4 *)u.flux_macro_W;
         F.macro = (d4 *)u.macro; F.macro_new = (d4 *)u.macro_new; F.residual = (d4 *)u.residual;
@@ -1169,6 +1197,9 @@ int ugks_step_plan_create(ugks_block *const *blocks, const ugks_block_buffers *b
             if (d.type < 0 || d.type > 9) { delete p; return ugks_rt::fail(-1, "step plan: bad BC type"); }
             if ((d.type == 0 || d.type == 5) && (d.other_block < 0 || d.other_block >= nblocks)) {
                 delete p; return ugks_rt::fail(-1, "step plan: exchange partner out of range");
+            }
+            if ((d.type == 0 || d.type == 5) && case_of_block && case_of_block[d.other_block] != F.cs) {
+                delete p; return ugks_rt::fail(-1, "step plan: exchange partner belongs to another case");
             }
             if (d.type == 8 && d.reflect_type == 2) {
                 delete p; return ugks_rt::fail(-2, "step plan: Cercignani-Lampis re-emission is not implemented");
@@ -1253,13 +1284,16 @@ int ugks_step_plan_create(ugks_block *const *blocks, const ugks_block_buffers *b
     (void)alloc((void **)&p->d_cell_off, sizeof(long long) * (nblocks + 1));
     (void)alloc((void **)&p->d_walls, sizeof(WallItem) * (walls.size() ? walls.size() : 1));
     (void)alloc((void **)&p->d_ghosts, sizeof(GhostItem) * ghosts.size());
-    (void)alloc((void **)&p->d_state, sizeof(DtState));
+    (void)alloc((void **)&p->d_state, sizeof(DtState) * ncases);
+    (void)alloc((void **)&p->d_case_first, sizeof(int) * (ncases + 1));
     (void)alloc((void **)&p->d_maxbits, sizeof(unsigned long long) * nblocks);
     (void)alloc((void **)&p->d_max_dt, sizeof(double) * nblocks);
     (void)alloc((void **)&p->d_fsum, sizeof(double) * kFS * (size_t)p->nfaces);
     (void)alloc((void **)&p->d_fstate, sizeof(double) * kFT * (size_t)p->nfaces);
     if (e != hipSuccess) { ugks_step_plan_destroy(p); return ugks_rt::hip_fail(e, "step plan: hipMalloc"); }
-    DtState st{dtd->dt, dtd->time, dtd->CFL, dtd->delta_dt, dtd->suggest_dt, dtd->Kn_eff, dtd->step, 0};
+    std::vector<DtState> st(ncases);
+    for (int c = 0; c < ncases; c++)
+        st[c] = DtState{dtd[c].dt, dtd[c].time, dtd[c].CFL, dtd[c].delta_dt, dtd[c].suggest_dt, dtd[c].Kn_eff, dtd[c].step, 0};
     std::vector<unsigned long long> zeros(nblocks, 0ull);
     if (e == hipSuccess) e = hipMemcpy(p->d_blocks, p->host_blocks.data(), bytes_blocks, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(p->d_face_off, face_off.data(), sizeof(long long) * (nblocks + 1), hipMemcpyHostToDevice);
@@ -1267,17 +1301,27 @@ int ugks_step_plan_create(ugks_block *const *blocks, const ugks_block_buffers *b
     if (e == hipSuccess && walls.size())
         e = hipMemcpy(p->d_walls, walls.data(), sizeof(WallItem) * walls.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(p->d_ghosts, ghosts.data(), sizeof(GhostItem) * ghosts.size(), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(p->d_state, &st, sizeof st, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(p->d_state, st.data(), sizeof(DtState) * ncases, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(p->d_case_first, case_first.data(), sizeof(int) * (ncases + 1), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(p->d_maxbits, zeros.data(), sizeof(unsigned long long) * nblocks, hipMemcpyHostToDevice);
     if (e != hipSuccess) { ugks_step_plan_destroy(p); return ugks_rt::hip_fail(e, "step plan: upload"); }
     *out = p;
     return 0;
 }
 
+int ugks_step_plan_create(ugks_block *const *blocks, const ugks_block_buffers *bufs, const ugks_bc_desc *bcs,
+                          int32_t nblocks, int32_t precision, const ugks_dt_desc *dtd, ugks_step_plan **out) {
+    return ugks_step_plan_create_batch(blocks, bufs, bcs, nblocks, nullptr, 1, precision, dtd, out);
+}
+
 int ugks_step_plan_destroy(ugks_step_plan *p) {
     if (!p) return 0;
+    if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    if (p->cap_stream) (void)hipStreamDestroy(p->cap_stream);
     void *ptrs[] = {p->d_blocks, p->d_face_off, p->d_cell_off, p->d_walls, p->d_ghosts, p->d_state,
-                    p->d_maxbits, p->d_max_dt, p->d_fsum, p->d_fstate};
+                    p->d_case_first, p->d_maxbits, p->d_max_dt, p->d_fsum, p->d_fstate};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     delete p;
@@ -1319,24 +1363,69 @@ int ugks_step_run_timed(ugks_step_plan *p, void *stream, int32_t nsteps, float *
     return rc;
 }
 
-int ugks_step_state(ugks_step_plan *p, void *stream, ugks_dt_desc *out, double *max_dt) {
+int ugks_step_states(ugks_step_plan *p, void *stream, ugks_dt_desc *out, double *max_dt) {
     if (!p || !out) return ugks_rt::fail(-1, "step state: bad arguments");
-    DtState st;
-    hipError_t e = hipMemcpyAsync(&st, p->d_state, sizeof st, hipMemcpyDeviceToHost, (hipStream_t)stream);
+    std::vector<DtState> st(p->ncases);
+    hipError_t e = hipMemcpyAsync(st.data(), p->d_state, sizeof(DtState) * p->ncases, hipMemcpyDeviceToHost,
+                                  (hipStream_t)stream);
     if (e == hipSuccess && max_dt)
         e = hipMemcpyAsync(max_dt, p->d_max_dt, sizeof(double) * p->nb, hipMemcpyDeviceToHost, (hipStream_t)stream);
     if (e == hipSuccess) e = hipStreamSynchronize((hipStream_t)stream);
     if (e != hipSuccess) return ugks_rt::hip_fail(e, "step state");
-    out->dt = st.dt; out->time = st.time; out->CFL = st.CFL; out->delta_dt = st.delta_dt;
-    out->suggest_dt = st.suggest_dt; out->Kn_eff = st.Kn; out->step = st.step; out->overshoot = st.overshoot;
+    for (int c = 0; c < p->ncases; c++) {
+        out[c].dt = st[c].dt; out[c].time = st[c].time; out[c].CFL = st[c].CFL; out[c].delta_dt = st[c].delta_dt;
+        out[c].suggest_dt = st[c].suggest_dt; out[c].Kn_eff = st[c].Kn; out[c].step = st[c].step;
+        out[c].overshoot = st[c].overshoot;
+    }
     return 0;
 }
 
-int ugks_step_set_dt(ugks_step_plan *p, void *stream, double dt) {
-    if (!p) return ugks_rt::fail(-1, "set dt: null plan");
-    hipError_t e = hipMemcpyAsync(&p->d_state->dt, &dt, sizeof(double), hipMemcpyHostToDevice, (hipStream_t)stream);
+int ugks_step_state(ugks_step_plan *p, void *stream, ugks_dt_desc *out, double *max_dt) {
+    if (!p || !out) return ugks_rt::fail(-1, "step state: bad arguments");
+    std::vector<ugks_dt_desc> all(p->ncases);
+    std::vector<double> md(max_dt ? p->nb : 0);
+    int rc = ugks_step_states(p, stream, all.data(), max_dt ? md.data() : nullptr);
+    if (rc) return rc;
+    *out = all[0];
+    if (max_dt) memcpy(max_dt, md.data(), sizeof(double) * p->nb);
+    return 0;
+}
+
+int ugks_step_set_dt_case(ugks_step_plan *p, void *stream, int32_t case_index, double dt) {
+    if (!p || case_index < 0 || case_index >= p->ncases) return ugks_rt::fail(-1, "set dt: bad arguments");
+    hipError_t e = hipMemcpyAsync(&p->d_state[case_index].dt, &dt, sizeof(double), hipMemcpyHostToDevice,
+                                  (hipStream_t)stream);
     if (e == hipSuccess) e = hipStreamSynchronize((hipStream_t)stream);
     return e == hipSuccess ? 0 : ugks_rt::hip_fail(e, "set dt");
+}
+
+int ugks_step_set_dt(ugks_step_plan *p, void *stream, double dt) { return ugks_step_set_dt_case(p, stream, 0, dt); }
+
+int ugks_step_run_graph(ugks_step_plan *p, void *stream, int32_t nsteps) {
+    if (!p || nsteps < 0) return ugks_rt::fail(-1, "step run graph: bad arguments");
+    if (!p->graph_exec) {
+        // one step captured on a stream of our own; the graph then runs on the caller's stream
+        hipError_t e = hipSuccess;
+        if (!p->cap_stream) e = hipStreamCreateWithFlags(&p->cap_stream, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipStreamBeginCapture(p->cap_stream, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) return ugks_rt::hip_fail(e, "step graph: begin capture");
+        int rc = p->precision == UGKS_FP64 ? dispatch_npt<double>(p, p->cap_stream, nullptr)
+                                           : dispatch_npt<float>(p, p->cap_stream, nullptr);
+        hipGraph_t g = nullptr;
+        e = hipStreamEndCapture(p->cap_stream, &g);
+        if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+        if (e != hipSuccess) return ugks_rt::hip_fail(e, "step graph: end capture");
+        hipGraphExec_t x = nullptr;
+        e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+        if (e != hipSuccess) { (void)hipGraphDestroy(g); return ugks_rt::hip_fail(e, "step graph: instantiate"); }
+        p->graph = g;
+        p->graph_exec = x;
+    }
+    for (int k = 0; k < nsteps; k++) {
+        hipError_t e = hipGraphLaunch(p->graph_exec, (hipStream_t)stream);
+        if (e != hipSuccess) return ugks_rt::hip_fail(e, "step graph: launch");
+    }
+    return 0;
 }
 
 int ugks_copy_kernel(void *stream, const void *src, void *dst, int64_t bytes) {

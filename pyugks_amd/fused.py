@@ -1,9 +1,12 @@
-"""The fused, batched step (ugks_step_* in include/ugks_hip.h) for a UGKSim.
+"""The fused, batched step (ugks_step_* in include/ugks_hip.h).
 
-``StepPlan(sim, precision)`` takes a simulation that the reference-shaped
-path has initialised (geometry, initial f, ghosts, first time step; the
-sequence of ugks_sim.py:52-92) and advances all of its blocks together:
-each step is six launches with the time step kept on the device.  With
+``StepPlan(sims, precision)`` takes one simulation, or a list of independent
+simulations (cases), that the reference-shaped path has initialised
+(geometry, initial f, ghosts, first time step; the sequence of
+ugks_sim.py:52-92) and advances all of their blocks together: each step is
+eight launches over every block of every case, with each case's time step
+kept on the device (one UGKSim.one_step per case, ugks_sim.py:407-501).
+With ``graph=True`` the step runs as replays of a captured HIP graph.  With
 ``precision="mixed"`` f and the per-velocity face fluxes are stored and
 computed in FP32; moments, the macroscopic update, heat fluxes, equilibrium
 moments and the time step stay FP64.
@@ -18,20 +21,28 @@ from . import lib as L
 
 
 class StepPlan:
-    def __init__(self, sim, precision: str = "fp64", stream=None):
-        self.sim = sim
+    KERNELS = ("k_faceA", "k_faceS", "k_faceC", "k_wall", "k_update", "k_dt", "k_dtfin", "k_ghost")
+
+    def __init__(self, sims, precision: str = "fp64", stream=None, graph: bool = False):
+        self.single = not isinstance(sims, (list, tuple))
+        self.sims = [sims] if self.single else list(sims)
+        self.sim = self.sims[0]
         self.stream = stream
         self.precision = precision
+        self.graph = graph
         mixed = precision == "mixed"
         if precision not in ("fp64", "mixed"):
             raise ValueError("precision must be 'fp64' or 'mixed'")
         lib = L.load()
-        nb = len(sim.blocks)
+        blocks = [(ci, b) for ci, sim in enumerate(self.sims) for b in sim.blocks]
+        first = np.cumsum([0] + [len(sim.blocks) for sim in self.sims])
+        nb = len(blocks)
         bufs = (L.BlockBuffers * nb)()
         bcs = (L.BCDesc * (4 * nb))()
         handles = (C.c_void_p * nb)()
+        case_of = (C.c_int32 * nb)(*[ci for ci, _ in blocks])
         rt = {"S": 0, "D": 1, "CL": 2}
-        for bi, b in enumerate(sim.blocks):
+        for bi, (ci, b) in enumerate(blocks):
             b.macro_new_D = L.DeviceArray((b.ni, b.nj, 4), fill=0.0)
             if mixed:
                 dist = (b.Ni, b.Nj, b.Nv, 2)
@@ -53,21 +64,30 @@ class StepPlan:
             u.wall_dist, u.time_step, u.flag = b.wall_dist_D.ptr, b.time_step_D.ptr, b.flag_D.ptr
             handles[bi] = b.prg.handle
             for k, x in enumerate(b.bc_list):
-                bcs[4 * bi + k] = L.BCDesc(int(x.type_of_BC), int(x.other_block), int(x.other_face),
+                other = int(x.other_block)
+                if other >= 0:
+                    other += int(first[ci])
+                bcs[4 * bi + k] = L.BCDesc(int(x.type_of_BC), other, int(x.other_face),
                                            int(x.orientation), int(x.flip_distribution), rt.get(x.reflect_type, 0))
-        g = sim.gdata
-        dt = L.DtDesc(g.dt, g.time, g.CFL, g.delta_dt, g.suggest_dt if g.suggest_dt else 0.0, g.Kn_eff,
-                      int(sim.step), 0)
+        dts = (L.DtDesc * len(self.sims))()
+        for ci, sim in enumerate(self.sims):
+            g = sim.gdata
+            dts[ci] = L.DtDesc(g.dt, g.time, g.CFL, g.delta_dt, g.suggest_dt if g.suggest_dt else 0.0, g.Kn_eff,
+                               int(sim.step), 0)
         h = C.c_void_p()
-        L.check(lib.ugks_step_plan_create(handles, bufs, bcs, nb, L.UGKS_MIXED if mixed else L.UGKS_FP64,
-                                          C.byref(dt), C.byref(h)), "ugks_step_plan_create")
+        L.check(lib.ugks_step_plan_create_batch(handles, bufs, bcs, nb, case_of, len(self.sims),
+                                                L.UGKS_MIXED if mixed else L.UGKS_FP64, dts, C.byref(h)),
+                "ugks_step_plan_create_batch")
         self.handle = h.value
-        self._keep = (bufs, bcs, handles)
+        self.blocks = [b for _, b in blocks]
+        self._keep = (bufs, bcs, handles, case_of, dts)
 
     def run(self, nsteps: int):
-        L.check(L.load().ugks_step_run(self.handle, self.stream, int(nsteps)), "ugks_step_run")
-
-    KERNELS = ("k_faceA", "k_faceS", "k_faceC", "k_wall", "k_update", "k_dt", "k_dtfin", "k_ghost")
+        lib = L.load()
+        if self.graph:
+            L.check(lib.ugks_step_run_graph(self.handle, self.stream, int(nsteps)), "ugks_step_run_graph")
+        else:
+            L.check(lib.ugks_step_run(self.handle, self.stream, int(nsteps)), "ugks_step_run")
 
     def run_timed(self, nsteps: int) -> dict:
         """Run nsteps with HIP events around each launch; per-kernel total ms."""
@@ -75,18 +95,25 @@ class StepPlan:
         L.check(L.load().ugks_step_run_timed(self.handle, self.stream, int(nsteps), ms), "ugks_step_run_timed")
         return dict(zip(self.KERNELS, [float(x) for x in ms]))
 
-    def state(self, update_sim: bool = True):
-        d = L.DtDesc()
-        md = np.zeros(len(self.sim.blocks))
-        L.check(L.load().ugks_step_state(self.handle, self.stream, C.byref(d),
-                                         md.ctypes.data_as(C.c_void_p)), "ugks_step_state")
+    def states(self, update_sim: bool = True):
+        """Time-step state of every case (and the per-block maximum time steps)."""
+        ds = (L.DtDesc * len(self.sims))()
+        md = np.zeros(len(self.blocks))
+        L.check(L.load().ugks_step_states(self.handle, self.stream, ds, md.ctypes.data_as(C.c_void_p)),
+                "ugks_step_states")
         if update_sim:
-            g = self.sim.gdata
-            g.dt, g.time, g.step = d.dt, d.time, d.step
-            self.sim.step = d.step
-            for b, m in zip(self.sim.blocks, md):
-                b.max_dt = float(m)
-        return d
+            k = 0
+            for sim, d in zip(self.sims, ds):
+                g = sim.gdata
+                g.dt, g.time, g.step = d.dt, d.time, d.step
+                sim.step = d.step
+                for b in sim.blocks:
+                    b.max_dt = float(md[k])
+                    k += 1
+        return list(ds)
+
+    def state(self, update_sim: bool = True):
+        return self.states(update_sim)[0]
 
     def f64(self, block):
         """f of one block as FP64 (converted from FP32 for the mixed step)."""
@@ -97,7 +124,7 @@ class StepPlan:
         return block.f_D.download(self.stream)
 
     def flag(self):
-        return max(float(b.flag_D.download(self.stream)[0]) for b in self.sim.blocks)
+        return max(float(b.flag_D.download(self.stream)[0]) for b in self.blocks)
 
     def close(self):
         if getattr(self, "handle", None):

@@ -63,11 +63,15 @@ KERNELS = {
 # fused / batched entry points: name -> argtypes (full list)
 FUSED = {
     "ugks_step_plan_create": [P, P, P, I, I, P, C.POINTER(P)],
+    "ugks_step_plan_create_batch": [P, P, P, I, P, I, I, P, C.POINTER(P)],
     "ugks_step_plan_destroy": [P],
     "ugks_step_run": [P, P, I],
+    "ugks_step_run_graph": [P, P, I],
     "ugks_step_run_timed": [P, P, I, P],
     "ugks_step_state": [P, P, P, P],
+    "ugks_step_states": [P, P, P, P],
     "ugks_step_set_dt": [P, P, D],
+    "ugks_step_set_dt_case": [P, P, I, D],
     "ugks_copy_kernel": [P, P, P, C.c_int64],
     "ugks_f64_to_f32": [P, P, P, C.c_int64],
     "ugks_f32_to_f64": [P, P, P, C.c_int64],

@@ -96,22 +96,48 @@ def test_fused_split_2x2_bitwise(hip):
         assert np.array_equal(b.f()[G:-G, G:-G], f1[G + i0:G + i0 + 8, G + j0:G + j0 + 6])
 
 
-def test_fused_batch_of_cases_matches_separate_plans(hip):
-    """Independent cases in separate plans stepped back to back agree with the
-    oracle; a plan never mixes cases (each case keeps its own time step)."""
+def test_fused_batch_of_cases_in_one_plan(hip):
+    """Independent cases of uneven size stepped by ONE plan (one launch per
+    kernel per step for all of them) each keep their own time step and agree
+    with the oracle stepping each case on its own."""
     from oracle import OracleSim
     from pyugks_amd.fused import StepPlan
     from pyugks_amd.sim import UGKSim
     gs = cases.cfg4_batch(ncases=3, sides=(8, 12), vgrids=(8, 12))
     sims = [UGKSim(copy.deepcopy(g)) for g in gs]
-    plans = [StepPlan(s, "fp64") for s in sims]
+    plan = StepPlan(sims, "fp64")
     oras = [OracleSim(copy.deepcopy(g), threads=4) for g in gs]
     for _ in range(3):
-        for p in plans:
-            p.run(1)
+        plan.run(1)
         for o in oras:
             o.one_step(True)
-    for s, p, o in zip(sims, plans, oras):
-        assert abs(p.state().dt - o.dt) <= TOL * o.dt
+        states = plan.states()
+        for d, o in zip(states, oras):
+            assert abs(d.dt - o.dt) <= TOL * o.dt
+            assert abs(d.time - o.time) <= TOL * o.time
+    assert len({round(d.dt, 15) for d in states}) == 3      # the cases really differ
+    for s, o in zip(sims, oras):
         assert rel_err(s.blocks[0].macro(), o.blocks[0]["macro"]) <= TOL
         assert rel_err(s.blocks[0].f(), o.blocks[0]["f"]) <= TOL
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_graph_replays_match_eager_steps(hip, precision):
+    """The step captured as a HIP graph and replayed gives bit-for-bit the
+    state of the eagerly launched step, replay after replay, with eager work
+    and synchronisations in between."""
+    from pyugks_amd.fused import StepPlan
+    from pyugks_amd.sim import UGKSim
+    g = cases.cavity(16, ("NC", 8), "WENO5", 0.01, nb=(2, 1))
+    eager_sim, graph_sim = UGKSim(copy.deepcopy(g)), UGKSim(copy.deepcopy(g))
+    eager = StepPlan(eager_sim, precision)
+    graph = StepPlan(graph_sim, precision, graph=True)
+    for rep in range(4):
+        eager.run(1 + rep % 2)
+        graph.run(1 + rep % 2)
+        de, dg = eager.state(), graph.state()
+        assert (de.dt, de.time, de.step) == (dg.dt, dg.time, dg.step), rep
+        for be, bg in zip(eager_sim.blocks, graph_sim.blocks):
+            assert np.array_equal(be.macro(), bg.macro()), rep
+            assert np.array_equal(eager.f64(be), graph.f64(bg)), rep
+            assert np.array_equal(be.Q_D.download(), bg.Q_D.download()), rep
