@@ -232,7 +232,9 @@ int ugks_step_plan_create_batch(ugks_block *const *blocks, const ugks_block_buff
                                 int32_t nblocks, const int32_t *case_of_block, int32_t ncases, int32_t precision,
                                 const ugks_dt_desc *dt, ugks_step_plan **out);
 int ugks_step_plan_destroy(ugks_step_plan *plan);
-/* nsteps full steps: flux, update, time step, BC refresh (ugks_sim.py:407-501) */
+/* nsteps full steps: flux, update, time step, BC refresh (ugks_sim.py:407-501).
+ * Each call first rederives the per-cell moments the plan keeps from f as it
+ * stands, so f may be changed by other calls between runs. */
 int ugks_step_run(ugks_step_plan *plan, void *stream, int32_t nsteps);
 /* the same steps as replays of a HIP graph of one step, captured on first use
  * (the step has no host round trip, so the graph is valid for the plan's life) */

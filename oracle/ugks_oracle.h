@@ -11,6 +11,20 @@
  * reference's evaluation order (velocity sums run v = 0..Nv-1 in order, as
  * the reference does with its default CL_local_size = 1).  The reference
  * file:line each function follows is cited beside it in ugks_oracle.c.
+ *
+ * Pins: tests/test_oracle_pins.py checks every function against what the
+ * mathematics fixes (closed forms, library quadrature, invariants, brute
+ * force); the reference ships no golden vectors to check against.  These
+ * parts have no such check and are PARITY UNPINNED (they follow a careful
+ * reading of the reference only):
+ *   - or_edgeExchange for rotated or flipped connections (ori/flip paths;
+ *     the pure E-W and N-S shifts are pinned by the 2x2 split test),
+ *   - or_edgeOutflow,
+ *   - or_adsorbingWallS_P2 (specular re-emission; the diffuse one is pinned),
+ *   - the momentum and energy columns of or_wallMassEnergyFluxes (the mass
+ *     columns are pinned by the zero-net-flux test),
+ *   - the clamp branches of or_adsorbingWall_P1 beyond the mass balance and
+ *     coverage bounds that are checked.
  */
 #ifndef UGKS_ORACLE_H
 #define UGKS_ORACLE_H

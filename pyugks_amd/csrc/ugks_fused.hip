@@ -17,11 +17,12 @@
 //   k_wall    one workgroup per diffuse/adsorbing wall face: reconstruction,
 //             accommodatingWallDist or adsorbingWall_P1 + S/D_P2, wallFlux,
 //             wallMassEnergyFluxes.
-//   k_update  one workgroup per cell: updateMacro + UGKS_update.
+//   k_update  one wavefront per cell: updateMacro + UGKS_update and the
+//             cell's moments for the next step's left/right face states.
 //   k_dt      one thread per cell: clFindDT and the per-block maximum.
 //   k_dtfin   one workgroup: the global time step with the reference's growth
 //             limit (ugks_sim.py:285-311, applied as one_step does, :430-446).
-//   k_ghost   one workgroup per ghost cell: every BC of every block
+//   k_ghost   one wavefront per ghost cell: every BC of every block
 //             (updateBC, ugks_block.py:621-665).
 //
 // Items of all blocks are laid out back to back; a workgroup finds its block
@@ -77,12 +78,31 @@ struct FBlock {
     int nS, nSj, ob_S, ob_W, nWi, wl;       // interior-face counts and offsets
     int flipidx[4];                         // exchange row flip from the corner check
     int cs;                                 // case of this block (its time-step state)
+    d4 *cmom;                               // [NI][NJ] moments of f per cell (all cells, ghosts included)
 };
 
 struct DtState {
     double dt, time, CFL, delta_dt, suggest_dt, Kn;
     int step, overshoot;
 };
+
+// Workgroups are dealt to the 8 XCDs round robin (workgroup id % 8), and
+// each XCD has its own L2.  Neighbouring faces and cells share f stencils and
+// fluxes, so the id is remapped to give every XCD one contiguous range of
+// the work instead of every eighth item.  The map is a bijection on
+// [0, n) for any n.
+#ifndef UGKS_XCD_REMAP
+#define UGKS_XCD_REMAP 1
+#endif
+__device__ __forceinline__ long long xcd_remap(long long id, long long n) {
+#if UGKS_XCD_REMAP
+    const long long x = id & 7, k = id >> 3, q = n >> 3, r = n & 7;
+    return x * q + (x < r ? x : r) + k;
+#else
+    (void)n;
+    return id;
+#endif
+}
 
 __device__ __forceinline__ int find_block(const long long *off, int nb, long long item) {
     int b = 0;
@@ -109,57 +129,75 @@ template <int N> __device__ __forceinline__ void block_sum(double (&a)[N], doubl
 }
 
 // ---- reconstruction (iFace, clUGKS.c:134-195) in precision T ---------------
-// quotient a / b from the hardware reciprocal estimate refined by two
-// Newton steps and one residual correction (within an ulp or two of the
-// correctly rounded quotient; b is a positive normal number here)
-__device__ __forceinline__ double qdiv(double a, double b) {
-    double r = __builtin_amdgcn_rcp(b);
-    double e = fma(-b, r, 1.0);
-    r = fma(r, e, r);
-    e = fma(-b, r, 1.0);
-    r = fma(r, e, r);
-    double q = a * r;
-    return fma(r, fma(-b, q, a), q);
-}
 __device__ __forceinline__ float rcp_T(float x) { return __builtin_amdgcn_rcpf(x); }
 
-// WENO5 slope.  The three nonlinear weights a_k = d_k / (eps + b_k)^2 enter
-// only through sum(a_k f_k) / sum(a_k); in double both sums are scaled by
-// prod_k (eps + b_k)^2 so that a single division remains ((eps + b_k)^2 >=
-// 1e-32, the products stay far from underflow), and the linear weights d_k
-// are folded into the candidate stencils.  In float that product can
-// underflow, so the weights use reciprocal estimates instead.
-__device__ __forceinline__ double weno5_T(double s0, double s1, double s2, double s3, double s4, double rdx) {
-    double t1 = s0 - 2.0 * s1 + s2, t2 = s0 - 4.0 * s1 + 3.0 * s2;
-    const double b0 = (13.0 / 12.0) * t1 * t1 + 0.25 * t2 * t2;
-    t1 = s1 - 2.0 * s2 + s3; t2 = s1 - s3;
-    const double b1 = (13.0 / 12.0) * t1 * t1 + 0.25 * t2 * t2;
-    t1 = s2 - 2.0 * s3 + s4; t2 = 3.0 * s2 - 4.0 * s3 + s4;
-    const double b2 = (13.0 / 12.0) * t1 * t1 + 0.25 * t2 * t2;
-    const double e0 = 1e-16 + b0, e1 = 1e-16 + b1, e2 = 1e-16 + b2;
+// WENO5 slope of both components (g, h) (clFunctions.c:268-312), written
+// in the first differences d_k = s_k - s_{k-1}:
+//   smoothness  12 b_0 = 13 (d2 - d1)^2 + 3 (3 d2 - d1)^2
+//               12 b_1 = 13 (d3 - d2)^2 + 3 (d2 + d3)^2
+//               12 b_2 = 13 (d4 - d3)^2 + 3 (3 d3 - d4)^2
+//   candidates  f_0 - s2 = -d1/3 + 5 d2/6,  f_1 - s2 = d2/6 + d3/3,
+//               f_2 - s2 = 2 d3/3 - d4/6
+// with eps scaled by 12 alongside (the weights only see the ratios), so that
+// (f_L - s2) is formed without cancelling against s2.  The nonlinear weights
+// a_k = d_k / (eps + b_k)^2 enter only through sum(a_k (f_k - s2)) /
+// sum(a_k): in double both sums are scaled by prod_k (eps + b_k)^2, leaving
+// one quotient per component, and the two quotients share one reciprocal.
+// In float that product can underflow, so the weights use reciprocal
+// estimates instead.
+struct WenoParts { double num, den; };
+__device__ __forceinline__ WenoParts weno5_parts(double s0, double s1, double s2, double s3, double s4) {
+    const double d1 = s1 - s0, d2 = s2 - s1, d3 = s3 - s2, d4 = s4 - s3;
+    const double u0 = d2 - d1, v0 = 3.0 * d2 - d1;
+    const double u1 = d3 - d2, v1 = d2 + d3;
+    const double u2 = d4 - d3, v2 = 3.0 * d3 - d4;
+    const double e0 = 1.2e-15 + (13.0 * u0 * u0 + 3.0 * v0 * v0);
+    const double e1 = 1.2e-15 + (13.0 * u1 * u1 + 3.0 * v1 * v1);
+    const double e2 = 1.2e-15 + (13.0 * u2 * u2 + 3.0 * v2 * v2);
     const double q0 = e0 * e0, q1 = e1 * e1, q2 = e2 * e2;
     const double p0 = q1 * q2, p1 = q0 * q2, p2 = q0 * q1;
-    // d_k f_k with d = (0.1, 0.6, 0.3)
-    const double f0 = (0.1 / 3.) * s0 - (0.7 / 6.) * s1 + (1.1 / 6.) * s2;
-    const double f1 = -0.1 * s1 + 0.5 * s2 + 0.2 * s3;
-    const double f2 = 0.1 * s2 + 0.25 * s3 - 0.05 * s4;
-    const double fL = qdiv(p0 * f0 + p1 * f1 + p2 * f2, 0.1 * p0 + 0.6 * p1 + 0.3 * p2);
-    return (fL - s2) * rdx;
+    // d_k (f_k - s2) with d = (0.1, 0.6, 0.3)
+    const double c0 = (-0.1 / 3.0) * d1 + (0.5 / 6.0) * d2;
+    const double c1 = 0.1 * d2 + 0.2 * d3;
+    const double c2 = 0.2 * d3 - 0.05 * d4;
+    return WenoParts{p0 * c0 + p1 * c1 + p2 * c2, 0.1 * p0 + 0.6 * p1 + 0.3 * p2};
 }
-__device__ __forceinline__ float weno5_T(float s0, float s1, float s2, float s3, float s4, float rdx) {
-    float t1 = s0 - 2.f * s1 + s2, t2 = s0 - 4.f * s1 + 3.f * s2;
-    const float b0 = (13.f / 12.f) * t1 * t1 + 0.25f * t2 * t2;
-    t1 = s1 - 2.f * s2 + s3; t2 = s1 - s3;
-    const float b1 = (13.f / 12.f) * t1 * t1 + 0.25f * t2 * t2;
-    t1 = s2 - 2.f * s3 + s4; t2 = 3.f * s2 - 4.f * s3 + s4;
-    const float b2 = (13.f / 12.f) * t1 * t1 + 0.25f * t2 * t2;
-    const float e0 = 1e-16f + b0, e1 = 1e-16f + b1, e2 = 1e-16f + b2;
+// the two slopes (f_L - s2) / dist of a (g, h) stencil
+__device__ __forceinline__ void weno5_pair(const P2<double> *s, double rdx, double &sgx, double &sgy) {
+    const WenoParts a = weno5_parts(s[0].x, s[1].x, s[2].x, s[3].x, s[4].x);
+    const WenoParts b = weno5_parts(s[0].y, s[1].y, s[2].y, s[3].y, s[4].y);
+    // 1 / (den_a den_b) from the hardware estimate and two Newton steps; the
+    // dens are >= 0.1 (1.2e-15)^4 and far below overflow, so is their product
+    const double D = a.den * b.den;
+    double r = __builtin_amdgcn_rcp(D);
+    double e = fma(-D, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-D, r, 1.0);
+    r = fma(r, e, r);
+    const double ra = r * b.den, rb = r * a.den;     // 1 / den_a, 1 / den_b
+    double qa = a.num * ra, qb = b.num * rb;
+    qa = fma(ra, fma(-a.den, qa, a.num), qa);        // one residual correction each
+    qb = fma(rb, fma(-b.den, qb, b.num), qb);
+    sgx = qa * rdx;
+    sgy = qb * rdx;
+}
+__device__ __forceinline__ float weno5_f(float s0, float s1, float s2, float s3, float s4) {
+    const float d1 = s1 - s0, d2 = s2 - s1, d3 = s3 - s2, d4 = s4 - s3;
+    const float u0 = d2 - d1, v0 = 3.f * d2 - d1;
+    const float u1 = d3 - d2, v1 = d2 + d3;
+    const float u2 = d4 - d3, v2 = 3.f * d3 - d4;
+    const float e0 = 1.2e-15f + (13.f * u0 * u0 + 3.f * v0 * v0);
+    const float e1 = 1.2e-15f + (13.f * u1 * u1 + 3.f * v1 * v1);
+    const float e2 = 1.2e-15f + (13.f * u2 * u2 + 3.f * v2 * v2);
     const float a0 = 0.1f * rcp_T(e0 * e0), a1 = 0.6f * rcp_T(e1 * e1), a2 = 0.3f * rcp_T(e2 * e2);
-    const float f0 = (1.f / 3.f) * s0 - (7.f / 6.f) * s1 + (11.f / 6.f) * s2;
-    const float f1 = -(1.f / 6.f) * s1 + (5.f / 6.f) * s2 + (1.f / 3.f) * s3;
-    const float f2 = (1.f / 3.f) * s2 + (5.f / 6.f) * s3 - (1.f / 6.f) * s4;
-    const float fL = (a0 * f0 + a1 * f1 + a2 * f2) * rcp_T(a0 + a1 + a2);
-    return (fL - s2) * rdx;
+    const float c0 = (-1.f / 3.f) * d1 + (5.f / 6.f) * d2;
+    const float c1 = (1.f / 6.f) * d2 + (1.f / 3.f) * d3;
+    const float c2 = (2.f / 3.f) * d3 - (1.f / 6.f) * d4;
+    return (a0 * c0 + a1 * c1 + a2 * c2) * rcp_T(a0 + a1 + a2);
+}
+__device__ __forceinline__ void weno5_pair(const P2<float> *s, float rdx, float &sgx, float &sgy) {
+    sgx = weno5_f(s[0].x, s[1].x, s[2].x, s[3].x, s[4].x) * rdx;
+    sgy = weno5_f(s[0].y, s[1].y, s[2].y, s[3].y, s[4].y) * rdx;
 }
 template <typename T> __device__ __forceinline__ T vanleer_T(T s1, T s2) {
     T a1 = s1 < T(0) ? -s1 : s1, a2 = s2 < T(0) ? -s2 : s2;
@@ -236,8 +274,7 @@ __device__ __forceinline__ void reconstruct(const Consts &C, const Stencil<T> &s
 #pragma unroll
         for (int k = 0; k < 5; k++) s[k] = sld<T>(st, delta ? k : 5 - k, vb);
         const T dist = (T)(delta ? g.dist[1] : g.dist[0]), rdx = (T)(delta ? g.rdist[1] : g.rdist[0]);
-        sg.x = weno5_T(s[0].x, s[1].x, s[2].x, s[3].x, s[4].x, rdx);
-        sg.y = weno5_T(s[0].y, s[1].y, s[2].y, s[3].y, s[4].y, rdx);
+        weno5_pair(s, rdx, sg.x, sg.y);
         ff.x = s[2].x + sg.x * dist;
         ff.y = s[2].y + sg.y * dist;
         if (fl) *fl = delta ? s[2] : s[3];
@@ -272,6 +309,8 @@ __device__ __forceinline__ void face_cells(const Consts &C, const Stencil<T> &st
 }
 
 // ============================================================ face passes ==
+// Faces are numbered so that neighbouring ids share stencil cells: S faces
+// column by column (j fastest), W faces row by row (i fastest).
 // Per interior face: (A) one wavefront reconstructs f at the face for every
 // velocity and sums the raw face moments (the conserved moments of
 // getFaceCons, the third-order ones the face heat flux of calcFaceQ needs,
@@ -280,7 +319,8 @@ __device__ __forceinline__ void face_cells(const Consts &C, const Stencil<T> &st
 // state (getPLR, initMacroFlux, calcFaceQ); (C) one wavefront rebuilds the
 // face values and forms the per-velocity flux (distFlux) and the macro flux
 // (macroFlux).  Face ids: per block, S faces then W faces (walls excluded).
-constexpr int kFS = 24;   // face sums per face
+constexpr int kNS = 15;   // face sums: 13 raw moments of the face values, the owner's heat flux
+constexpr int kFS = 16;   // face sums per face (padded)
 constexpr int kFT = 32;   // face state per face
 
 struct FaceLoc { int face, mi, mj, gi, gj; };
@@ -288,7 +328,7 @@ __device__ __forceinline__ FaceLoc face_loc(const FBlock &B, long long loc) {
     const Consts &C = B.c;
     FaceLoc f;
     if (loc < B.nS) { f.face = SOUTH; f.mi = (int)(loc / B.nSj); f.mj = (int)(loc % B.nSj) + B.ob_S; }
-    else { long long r = loc - B.nS; f.face = WEST; f.mi = (int)(r / C.nj) + B.ob_W; f.mj = (int)(r % C.nj); }
+    else { long long r = loc - B.nS; f.face = WEST; f.mj = (int)(r / B.nWi); f.mi = (int)(r % B.nWi) + B.ob_W; }
     f.gi = f.mi + C.G;
     f.gj = f.mj + C.G;
     return f;
@@ -297,7 +337,7 @@ __device__ __forceinline__ FaceLoc face_loc(const FBlock &B, long long loc) {
 template <typename T>
 FACE_KERNEL k_faceA(const FBlock *__restrict__ blocks, const long long *__restrict__ off,
                                               int nb, long long nfaces, double *__restrict__ fsum) {
-    const long long fid = (long long)blockIdx.x * kFWaves + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const long long fid = xcd_remap(blockIdx.x, gridDim.x) * kFWaves + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (fid >= nfaces) return;
     const int lane = threadIdx.x & 63;
     const int b = find_block(off, nb, fid);
@@ -310,42 +350,34 @@ FACE_KERNEL k_faceA(const FBlock *__restrict__ blocks, const long long *__restri
     const bool wantQ = (L.face == SOUTH) && (L.mj < C.nj);
     const d4 cm = wantQ ? B.macro[(size_t)L.mi * C.nj + L.mj] : mk4(0, 0, 0, 1);
     const double rK = 1.0 / C.K;
-    double a[23];
+    double a[kNS];
 #pragma unroll
-    for (int q = 0; q < 23; q++) a[q] = 0.0;
+    for (int q = 0; q < kNS; q++) a[q] = 0.0;
     for (int v = lane; v < C.NV; v += 64) {
         const d2 uv = face_velocity(C, v, n);
         P2<T> ff, sg, fl, fr;
-#if UGKS_FACE_LR_LOAD
-        reconstruct<T>(C, stn, g, v, upwind(uv.x), ff, sg);
-        face_cells<T>(C, stn, v, fl, fr);
-#else
         reconstruct<T>(C, stn, g, v, upwind(uv.x), ff, sg, &fl, &fr);
-#endif
-        const double u = uv.x, w = uv.y, gx = (double)ff.x, hx = (double)ff.y, u2 = u * u + w * w;
+        const double u = uv.x, w = uv.y, gx = (double)ff.x, hx = (double)ff.y;
         const double ug = u * gx, wg = w * gx, uug = u * ug, wwg = w * wg;
         a[0] += gx; a[1] += ug; a[2] += wg;
         a[3] += uug; a[4] += wwg; a[5] += u * wg;
         a[6] += u * uug; a[7] += w * wwg; a[8] += u * wwg; a[9] += w * uug;
         a[10] += hx; a[11] += u * hx; a[12] += w * hx;
-        double x = (double)fl.x, y = (double)fl.y;
-        a[13] += x; a[14] += u * x; a[15] += w * x; a[16] += 0.5 * (u2 * x + y);
-        x = (double)fr.x; y = (double)fr.y;
-        a[17] += x; a[18] += u * x; a[19] += w * x; a[20] += 0.5 * (u2 * x + y);
         if (wantQ) {
             const d2 e = C.quad[v];
+            const double x = (double)fr.x, y = (double)fr.y;
             const double cx = e.x - cm.s1, cy = e.y - cm.s2, t = (cx * cx + cy * cy) * x + y * rK;
-            a[21] += 0.5 * (cx * t);
-            a[22] += 0.5 * (cy * t);
+            a[13] += 0.5 * (cx * t);
+            a[14] += 0.5 * (cy * t);
         }
     }
 #pragma unroll
-    for (int q = 0; q < 23; q++) a[q] = wave_sum(a[q]);
+    for (int q = 0; q < kNS; q++) a[q] = wave_sum(a[q]);
     if (lane == 0) {
         double *o = fsum + fid * kFS;
 #pragma unroll
-        for (int q = 0; q < 23; q++) o[q] = a[q];
-        if (wantQ) B.Q[(size_t)L.mi * C.nj + L.mj] = mk2(a[21], a[22]);
+        for (int q = 0; q < kNS; q++) o[q] = a[q];
+        if (wantQ) B.Q[(size_t)L.mi * C.nj + L.mj] = mk2(a[13], a[14]);
     }
 }
 
@@ -365,7 +397,11 @@ __global__ void __launch_bounds__(kT) k_faceS(const FBlock *__restrict__ blocks,
     const long long step = L.face == SOUTH ? 1 : C.NJ;
     const double *S = fsum + fid * kFS;
     const d4 W = mk4(S[0], S[1], S[2], 0.5 * ((S[3] + S[4]) + S[10]));
-    const d4 WL = mk4(S[13], S[14], S[15], S[16]), WR = mk4(S[17], S[18], S[19], S[20]);
+    // getAL / getAR: the moments of the cells below and above the face in face coordinates
+    const d2 nf = B.normal[own * 2 + L.face];
+    const d4 ML = B.cmom[(long long)own - step], MR = B.cmom[own];
+    const d2 uL = to_local(mk2(ML.s1, ML.s2), nf), uR = to_local(mk2(MR.s1, MR.s2), nf);
+    const d4 WL = mk4(ML.s0, uL.x, uL.y, ML.s3), WR = mk4(MR.s0, uR.x, uR.y, MR.s3);
     const d4 prim = to_primary(C, W);
     const d2 ms = B.mid_side[own * 2 + L.face];
     const d4 aL = micro_slope(C, prim, (W - WL) / length(ms - B.centre[(long long)own - step]));
@@ -403,7 +439,7 @@ FACE_KERNEL k_faceC(const FBlock *__restrict__ blocks, const long long *__restri
                                               int nb, long long nfaces, const double *__restrict__ fstate) {
     __shared__ double tab[kFWaves][128];
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const long long fid = (long long)blockIdx.x * kFWaves + wv;
+    const long long fid = xcd_remap(blockIdx.x, gridDim.x) * kFWaves + wv;
     if (fid >= nfaces) return;
     const int lane = threadIdx.x & 63;
     const int b = find_block(off, nb, fid);
@@ -742,12 +778,49 @@ __global__ void __launch_bounds__(kT) k_wall(const FBlock *__restrict__ blocks, 
     (void)sc;
 }
 
+// ======================================================= cell moments ====
+// M = sum_v (g, e_x g, e_y g, (|e|^2 g + h) / 2) of one cell in the global
+// frame: the getAL/getAR moments of its faces after rotation into the face
+// frame.  Kept for every cell (ghosts included) by the update and the ghost
+// fill; k_cellmom recomputes them all at the start of each run.
+template <typename T>
+__device__ __forceinline__ void add_moments(const Consts &C, int v, T gx, T hx, double (&m)[4]) {
+    const d2 e = C.quad[v];
+    const double g = (double)gx, h = (double)hx;
+    m[0] += g; m[1] += e.x * g; m[2] += e.y * g; m[3] += 0.5 * ((e.x * e.x + e.y * e.y) * g + h);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kT) k_cellmom(const FBlock *__restrict__ blocks, const long long *__restrict__ off,
+                                                int nb, long long nall) {
+    const long long item = xcd_remap(blockIdx.x, gridDim.x) * kWavesT + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (item >= nall) return;
+    const int lane = threadIdx.x & 63;
+    const int b = find_block(off, nb, item);
+    const FBlock &B = blocks[b];
+    const Consts &C = B.c;
+    const long long cell = item - off[b];
+    const P2<T> *F = (const P2<T> *)B.f + cell * C.NV;
+    double m[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int v = lane; v < C.NV; v += 64) {
+        const P2<T> x = F[v];
+        add_moments<T>(C, v, x.x, x.y, m);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) m[q] = wave_sum(m[q]);
+    if (lane == 0) B.cmom[cell] = mk4(m[0], m[1], m[2], m[3]);
+}
+
 // ============================================================ k_update ====
-template <typename T, int NPT>
+// one wavefront per cell: updateMacro, UGKS_update and the cell's moments
+template <typename T>
 __global__ void __launch_bounds__(kT) k_update(const FBlock *__restrict__ blocks, const long long *__restrict__ off,
-                                               int nb, const DtState *__restrict__ ds) {
-    __shared__ double tab[4][64];
-    const long long item = blockIdx.x;
+                                               int nb, long long ncells, const DtState *__restrict__ ds) {
+    __shared__ double tabs[kWavesT][4][64];
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const long long item = xcd_remap(blockIdx.x, gridDim.x) * kWavesT + wv;
+    if (item >= ncells) return;
+    const int lane = threadIdx.x & 63;
     const int b = find_block(off, nb, item);
     const FBlock &B = blocks[b];
     const Consts &C = B.c;
@@ -760,7 +833,7 @@ __global__ void __launch_bounds__(kT) k_update(const FBlock *__restrict__ blocks
     const double A = B.area[c0];
     const d4 dW = ((B.fmS[c0] - B.fmS[c0 + 1]) + B.fmW[c0]) - B.fmW[c0 + C.NJ];
     const d4 p = to_primary(C, to_conserved(C, pold) + dW / A);
-    if (threadIdx.x == 0) {
+    if (lane == 0) {
         B.macro_new[cell] = p;
         B.residual[cell] = pold;
     }
@@ -768,15 +841,19 @@ __global__ void __launch_bounds__(kT) k_update(const FBlock *__restrict__ blocks
     const double tau_old = relax_time(C, pold), tau = relax_time(C, p);
     const d2 Q = B.Q[cell];
     const bool sep = C.nut > 0;
+    double (*tab)[64] = tabs[wv];
     if (sep) {
-        for (int k = threadIdx.x; k < 2 * (C.nut + C.nvt); k += kT) {
+        for (int k = lane; k < 2 * (C.nut + C.nvt); k += 64) {
             const int which = k >= C.nut + C.nvt, r = which ? k - (C.nut + C.nvt) : k;
             const d4 &pp = which ? p : pold;
             const double a = r < C.nut ? C.qu[r] - pp.s1 : C.qv[r - C.nut] - pp.s2;
             tab[2 * which + (r < C.nut ? 0 : 1)][r < C.nut ? r : r - C.nut] = exp(-pp.s3 * (a * a));
         }
-        __syncthreads();
     }
+    // the tables are private to this wavefront: order its LDS writes before its reads
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     const double rlo = pold.s0 * (pold.s3 / kPi), rln = p.s0 * (p.s3 / kPi);
     const double so = 0.8 * (1 - C.Pr) * (pold.s3 * pold.s3) / pold.s0, sn = 0.8 * (1 - C.Pr) * (p.s3 * p.s3) / p.s0;
     const double hfo = C.K / (2.0 * pold.s3), hfn = C.K / (2.0 * p.s3);
@@ -788,48 +865,48 @@ __global__ void __launch_bounds__(kT) k_update(const FBlock *__restrict__ blocks
     const AS1 V2<T> *fS = (const AS1 V2<T> *)B.fluxS + c0 * NV;
     const AS1 V2<T> *fW = (const AS1 V2<T> *)B.fluxW + c0 * NV;
     const unsigned nS = (unsigned)NV, nE = (unsigned)(C.NJ * NV);
-    // tensor indices of v = threadIdx.x + 256 k, advanced without integer division
+    // tensor indices of v = lane + 64 k, advanced without integer division
     int iu = 0, jv = 0, diu = 0, djv = 0;
-    if (sep) { iu = threadIdx.x / C.nvt; jv = threadIdx.x - iu * C.nvt; diu = kT / C.nvt; djv = kT - diu * C.nvt; }
+    if (sep) { iu = lane / C.nvt; jv = lane - iu * C.nvt; diu = 64 / C.nvt; djv = 64 - diu * C.nvt; }
     bool nan_seen = false;
-#pragma unroll
-    for (int k = 0; k < NPT; k++) {
-        const int v = threadIdx.x + kT * k;
-        if (v < C.NV) {
-            const d2 e = C.quad[v];
-            const double w = C.weight[v];
-            const double ox = e.x - pold.s1, oy = e.y - pold.s2, nx = e.x - p.s1, ny = e.y - p.s2;
-            const double co = ox * ox + oy * oy, cn = nx * nx + ny * ny;
-            double Eo, En;
-            if (sep) {
-                Eo = tab[0][iu] * tab[1][jv];
-                En = tab[2][iu] * tab[3][jv];
-            } else {
-                Eo = exp(-pold.s3 * co);
-                En = exp(-p.s3 * cn);
-            }
-            const double gO = w * (rlo * Eo), hO = gO * hfo;
-            const double shO = so * (ox * Q.x + oy * Q.y), ptO = 2 * pold.s3 * co;
-            const double feqOg = gO + shO * (ptO - 4) * gO, feqOh = hO + shO * (ptO - 2) * hO;
-            const double gN = w * (rln * En), hN = gN * hfn;
-            const double shN = sn * (nx * Q.x + ny * Q.y), ptN = 2 * p.s3 * cn;
-            const double feqNg = gN + shN * (ptN - 4) * gN, feqNh = hN + shN * (ptN - 2) * hN;
-            const V2<T> fo = fc[v], sS = fS[v], nS_ = fS[v + nS], wW = fW[v], eW = fW[v + nE];
-            const double flg = (((double)sS.x - (double)nS_.x) + (double)wW.x - (double)eW.x) * rA;
-            const double flh = (((double)sS.y - (double)nS_.y) + (double)wW.y - (double)eW.y) * rA;
-            const double fg = (double)fo.x, fh = (double)fo.y;
-            const double ng = (fg + flg + (htn * feqNg + hto * (feqOg - fg))) * rden;
-            const double nh = (fh + flh + (htn * feqNh + hto * (feqOh - fh))) * rden;
-            const T tg = (T)ng, th = (T)nh;
-            if (isnan(tg) || isnan(th)) nan_seen = true;
-            fc[v] = V2<T>{tg, th};
-        }
+    double mom[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int v = lane; v < C.NV; v += 64) {
+        const d2 e = C.quad[v];
+        const double w = C.weight[v];
+        const double ox = e.x - pold.s1, oy = e.y - pold.s2, nx = e.x - p.s1, ny = e.y - p.s2;
+        const double co = ox * ox + oy * oy, cn = nx * nx + ny * ny;
+        double Eo, En;
         if (sep) {
+            Eo = tab[0][iu] * tab[1][jv];
+            En = tab[2][iu] * tab[3][jv];
             iu += diu; jv += djv;
             if (jv >= C.nvt) { jv -= C.nvt; iu++; }
+        } else {
+            Eo = exp(-pold.s3 * co);
+            En = exp(-p.s3 * cn);
         }
+        const double gO = w * (rlo * Eo), hO = gO * hfo;
+        const double shO = so * (ox * Q.x + oy * Q.y), ptO = 2 * pold.s3 * co;
+        const double feqOg = gO + shO * (ptO - 4) * gO, feqOh = hO + shO * (ptO - 2) * hO;
+        const double gN = w * (rln * En), hN = gN * hfn;
+        const double shN = sn * (nx * Q.x + ny * Q.y), ptN = 2 * p.s3 * cn;
+        const double feqNg = gN + shN * (ptN - 4) * gN, feqNh = hN + shN * (ptN - 2) * hN;
+        const V2<T> fo = fc[v], sS = fS[v], nS_ = fS[v + nS], wW = fW[v], eW = fW[v + nE];
+        const double flg = (((double)sS.x - (double)nS_.x) + (double)wW.x - (double)eW.x) * rA;
+        const double flh = (((double)sS.y - (double)nS_.y) + (double)wW.y - (double)eW.y) * rA;
+        const double fg = (double)fo.x, fh = (double)fo.y;
+        const double ng = (fg + flg + (htn * feqNg + hto * (feqOg - fg))) * rden;
+        const double nh = (fh + flh + (htn * feqNh + hto * (feqOh - fh))) * rden;
+        const T tg = (T)ng, th = (T)nh;
+        if (isnan(tg) || isnan(th)) nan_seen = true;
+        fc[v] = V2<T>{tg, th};
+        // the new cell moments (the getAL / getAR inputs of the next step)
+        add_moments<T>(C, v, tg, th, mom);
     }
     if (nan_seen) B.flag[0] = ERR_NAN;
+#pragma unroll
+    for (int q = 0; q < 4; q++) mom[q] = wave_sum(mom[q]);
+    if (lane == 0) B.cmom[c0] = mk4(mom[0], mom[1], mom[2], mom[3]);
 }
 
 // ============================================================ k_dt =========
@@ -916,8 +993,10 @@ struct GhostItem { int block, face; long long start; };
 
 template <typename T>
 __global__ void __launch_bounds__(kT) k_ghost(const FBlock *__restrict__ blocks, const GhostItem *__restrict__ items,
-                                              int nitems) {
-    const long long item = blockIdx.x;
+                                              int nitems, long long nghosts) {
+    const long long item = (long long)blockIdx.x * kWavesT + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (item >= nghosts) return;
+    const int lane = threadIdx.x & 63;
     int it = 0;
     while (it + 1 < nitems && items[it + 1].start <= item) it++;
     const GhostItem GI = items[it];
@@ -946,6 +1025,17 @@ __global__ void __launch_bounds__(kT) k_ghost(const FBlock *__restrict__ blocks,
     const size_t NV = C.NV;
     const size_t dst = ((size_t)gi * C.NJ + gj) * NV;
     const int type = B.bc[face];
+    // the ghost cell's moments for the faces it borders (uniform exits: the whole workgroup reaches these)
+    double m[4] = {0.0, 0.0, 0.0, 0.0};
+    auto put = [&](int v, P2<T> x) {
+        ((P2<T> *)B.f)[dst + v] = x;
+        add_moments<T>(C, v, x.x, x.y, m);
+    };
+    auto finish = [&]() {
+#pragma unroll
+        for (int q = 0; q < 4; q++) m[q] = wave_sum(m[q]);
+        if (lane == 0) B.cmom[(size_t)gi * C.NJ + gj] = mk4(m[0], m[1], m[2], m[3]);
+    };
     if (type == 0 || type == 5) {           // edgeExchange (clGhostRoutines.c:407-538)
         const FBlock &O = blocks[B.other[face]];
         int iB = gi, jB = gj;
@@ -975,18 +1065,20 @@ __global__ void __launch_bounds__(kT) k_ghost(const FBlock *__restrict__ blocks,
         if (iB < 0 || iB >= NIB || jB < 0 || jB >= NJB) return;
         const size_t src = ((size_t)iB * NJB + jB) * NV;
         const int fl = B.flip[face];
-        for (int v = threadIdx.x; v < C.NV; v += kT) {
+        for (int v = lane; v < C.NV; v += 64) {
             int v2 = v;
             if (fl == FLIP_NS) v2 = (int)C.mNS[v];
             else if (fl == FLIP_D) v2 = (int)C.mD[v];
             else if (fl == FLIP_EW) v2 = (int)C.mEW[v];
-            ((P2<T> *)B.f)[dst + v] = ld<T>(O.f, src + v2);
+            put(v, ld<T>(O.f, src + v2));
         }
+        finish();
         return;
     }
     if (type == 1) {                         // edgeExtrapolate
         const size_t src = ((size_t)ei * C.NJ + ej) * NV;
-        for (int v = threadIdx.x; v < C.NV; v += kT) ((P2<T> *)B.f)[dst + v] = ld<T>(B.f, src + v);
+        for (int v = lane; v < C.NV; v += 64) put(v, ld<T>(B.f, src + v));
+        finish();
         return;
     }
     if (type == 3 || type == 9) {            // edgeMirror / edgeBounceBack
@@ -998,10 +1090,11 @@ __global__ void __launch_bounds__(kT) k_ghost(const FBlock *__restrict__ blocks,
         default: mi = 2 * C.G - gi - 1; mj = gj; break;
         }
         const size_t src = ((size_t)mi * C.NJ + mj) * NV;
-        for (int v = threadIdx.x; v < C.NV; v += kT) {
+        for (int v = lane; v < C.NV; v += 64) {
             int mv = (int)(type == 9 ? C.mD[v] : (ns ? C.mNS[v] : C.mEW[v]));
-            ((P2<T> *)B.f)[dst + v] = ld<T>(B.f, src + mv);
+            put(v, ld<T>(B.f, src + mv));
         }
+        finish();
         return;
     }
     if (type == 4 || type == 8) {            // edgeConstGrad
@@ -1015,10 +1108,11 @@ __global__ void __launch_bounds__(kT) k_ghost(const FBlock *__restrict__ blocks,
         }
         const size_t s0 = ((size_t)ei0 * C.NJ + ej0) * NV, s1 = ((size_t)ei * C.NJ + ej) * NV;
         const T mm = (T)m;
-        for (int v = threadIdx.x; v < C.NV; v += kT) {
+        for (int v = lane; v < C.NV; v += 64) {
             P2<T> f0 = ld<T>(B.f, s0 + v), f1 = ld<T>(B.f, s1 + v);
-            ((P2<T> *)B.f)[dst + v] = P2<T>{f1.x + mm * (f1.x - f0.x), f1.y + mm * (f1.y - f0.y)};
+            put(v, P2<T>{f1.x + mm * (f1.x - f0.x), f1.y + mm * (f1.y - f0.y)});
         }
+        finish();
         return;
     }
     // edgeConstant (2), edgeInflow (6), edgeOutflow (7): a Shakhov equilibrium
@@ -1036,10 +1130,11 @@ __global__ void __launch_bounds__(kT) k_ghost(const FBlock *__restrict__ blocks,
         if (type == 6) { pr.s0 = wp.s0; pr.s3 = wp.s3; }
         else pr.s3 = pr.s0 / (wp.s0 / wp.s3);
     }
-    for (int v = threadIdx.x; v < C.NV; v += kT) {
+    for (int v = lane; v < C.NV; v += 64) {
         d2 fe = shakhov_eq(C, pr, Q, C.quad[v], v);
-        ((P2<T> *)B.f)[dst + v] = P2<T>{(T)fe.x, (T)fe.y};
+        put(v, P2<T>{(T)fe.x, (T)fe.y});
     }
+    finish();
 }
 
 // FP64 -> FP32 conversion of f (the mixed-precision state starts from the FP64 initial state)
@@ -1064,7 +1159,9 @@ struct ugks_step_plan {
     int nb, precision, npt;
     int device;
     FBlock *d_blocks;
-    long long *d_face_off, *d_cell_off;
+    long long *d_face_off, *d_cell_off, *d_all_off;   // prefix offsets: interior faces, cells, all cells
+    long long nall;
+    d4 *d_cmom;                                         // cell moments of every block, back to back
     WallItem *d_walls;
     GhostItem *d_ghosts;
     int nwall_items, nghost_items;
@@ -1104,8 +1201,8 @@ template <typename T, int NPT> int launch_step(ugks_step_plan *p, hipStream_t s,
         hipLaunchKernelGGL((k_wall<T, NPT>), dim3((unsigned)p->nwalls), dim3(kT), 0, s, p->d_blocks, p->d_walls,
                            p->nwall_items, p->d_state);
     if (ev) (void)hipEventRecord(ev[4], s);
-    hipLaunchKernelGGL((k_update<T, NPT>), dim3((unsigned)p->ncells), dim3(kT), 0, s, p->d_blocks, p->d_cell_off, p->nb,
-                       p->d_state);
+    hipLaunchKernelGGL((k_update<T>), dim3((unsigned)((p->ncells + kWavesT - 1) / kWavesT)), dim3(kT), 0, s,
+                       p->d_blocks, p->d_cell_off, p->nb, p->ncells, p->d_state);
     if (ev) (void)hipEventRecord(ev[5], s);
     hipLaunchKernelGGL(k_dt, dim3((unsigned)((p->ncells + kT - 1) / kT)), dim3(kT), 0, s, p->d_blocks, p->d_cell_off,
                        p->nb, p->d_state, p->d_maxbits);
@@ -1114,10 +1211,20 @@ template <typename T, int NPT> int launch_step(ugks_step_plan *p, hipStream_t s,
                        p->d_max_dt, (const int *)p->d_case_first, p->ncases);
     if (ev) (void)hipEventRecord(ev[7], s);
     if (p->nghosts > 0)
-        hipLaunchKernelGGL((k_ghost<T>), dim3((unsigned)p->nghosts), dim3(kT), 0, s, p->d_blocks, p->d_ghosts,
-                           p->nghost_items);
+        hipLaunchKernelGGL((k_ghost<T>), dim3((unsigned)((p->nghosts + kWavesT - 1) / kWavesT)), dim3(kT), 0, s,
+                           p->d_blocks, p->d_ghosts, p->nghost_items, p->nghosts);
     if (ev) (void)hipEventRecord(ev[8], s);
     return ugks_rt::check_launch("ugks_step_run");
+}
+
+// moments of every cell from f as it stands (f may have been changed between runs)
+template <typename T> int refresh_moments(ugks_step_plan *p, hipStream_t s) {
+    hipLaunchKernelGGL((k_cellmom<T>), dim3((unsigned)((p->nall + kWavesT - 1) / kWavesT)), dim3(kT), 0, s, p->d_blocks,
+                       p->d_all_off, p->nb, p->nall);
+    return ugks_rt::check_launch("ugks_step: cell moments");
+}
+int refresh(ugks_step_plan *p, hipStream_t s) {
+    return p->precision == UGKS_FP64 ? refresh_moments<double>(p, s) : refresh_moments<float>(p, s);
 }
 
 template <typename T> int dispatch_npt(ugks_step_plan *p, hipStream_t s, hipEvent_t *ev) {
@@ -1163,7 +1270,7 @@ int ugks_step_plan_create_batch(ugks_block *const *blocks, const ugks_block_buff
     p->ncases = ncases;
     p->precision = precision;
     int maxnv = 0;
-    std::vector<long long> face_off(nblocks + 1, 0), cell_off(nblocks + 1, 0);
+    std::vector<long long> face_off(nblocks + 1, 0), cell_off(nblocks + 1, 0), all_off(nblocks + 1, 0);
     std::vector<WallItem> walls;
     std::vector<GhostItem> ghosts;
     long long nw = 0, ng = 0;
@@ -1219,6 +1326,7 @@ int ugks_step_plan_create_batch(ugks_block *const *blocks, const ugks_block_buff
         F.wl = C.ni > C.nj ? C.ni : C.nj;
         face_off[b + 1] = face_off[b] + F.nS + (long long)F.nWi * C.nj;
         cell_off[b + 1] = cell_off[b] + (long long)C.ni * C.nj;
+        all_off[b + 1] = all_off[b] + (long long)C.NI * C.NJ;
         for (int k = 0; k < 4; k++) {
             int len = (k == GNORTH || k == GSOUTH) ? C.ni : C.nj;
             if (wallf[k]) { walls.push_back(WallItem{b, k, nw}); nw += len; }
@@ -1272,6 +1380,7 @@ int ugks_step_plan_create_batch(ugks_block *const *blocks, const ugks_block_buff
         }
     p->nfaces = face_off[nblocks];
     p->ncells = cell_off[nblocks];
+    p->nall = all_off[nblocks];
     p->nwalls = nw;
     p->nghosts = ng;
     p->nwall_items = (int)walls.size();
@@ -1282,6 +1391,8 @@ int ugks_step_plan_create_batch(ugks_block *const *blocks, const ugks_block_buff
     (void)alloc((void **)&p->d_blocks, bytes_blocks);
     (void)alloc((void **)&p->d_face_off, sizeof(long long) * (nblocks + 1));
     (void)alloc((void **)&p->d_cell_off, sizeof(long long) * (nblocks + 1));
+    (void)alloc((void **)&p->d_all_off, sizeof(long long) * (nblocks + 1));
+    (void)alloc((void **)&p->d_cmom, sizeof(d4) * (size_t)p->nall);
     (void)alloc((void **)&p->d_walls, sizeof(WallItem) * (walls.size() ? walls.size() : 1));
     (void)alloc((void **)&p->d_ghosts, sizeof(GhostItem) * ghosts.size());
     (void)alloc((void **)&p->d_state, sizeof(DtState) * ncases);
@@ -1295,9 +1406,12 @@ int ugks_step_plan_create_batch(ugks_block *const *blocks, const ugks_block_buff
     for (int c = 0; c < ncases; c++)
         st[c] = DtState{dtd[c].dt, dtd[c].time, dtd[c].CFL, dtd[c].delta_dt, dtd[c].suggest_dt, dtd[c].Kn_eff, dtd[c].step, 0};
     std::vector<unsigned long long> zeros(nblocks, 0ull);
+    if (e == hipSuccess)
+        for (int b = 0; b < nblocks; b++) p->host_blocks[b].cmom = p->d_cmom + all_off[b];
     if (e == hipSuccess) e = hipMemcpy(p->d_blocks, p->host_blocks.data(), bytes_blocks, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(p->d_face_off, face_off.data(), sizeof(long long) * (nblocks + 1), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(p->d_cell_off, cell_off.data(), sizeof(long long) * (nblocks + 1), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(p->d_all_off, all_off.data(), sizeof(long long) * (nblocks + 1), hipMemcpyHostToDevice);
     if (e == hipSuccess && walls.size())
         e = hipMemcpy(p->d_walls, walls.data(), sizeof(WallItem) * walls.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(p->d_ghosts, ghosts.data(), sizeof(GhostItem) * ghosts.size(), hipMemcpyHostToDevice);
@@ -1320,8 +1434,8 @@ int ugks_step_plan_destroy(ugks_step_plan *p) {
     if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
     if (p->graph) (void)hipGraphDestroy(p->graph);
     if (p->cap_stream) (void)hipStreamDestroy(p->cap_stream);
-    void *ptrs[] = {p->d_blocks, p->d_face_off, p->d_cell_off, p->d_walls, p->d_ghosts, p->d_state,
-                    p->d_case_first, p->d_maxbits, p->d_max_dt, p->d_fsum, p->d_fstate};
+    void *ptrs[] = {p->d_blocks, p->d_face_off, p->d_cell_off, p->d_all_off, p->d_cmom, p->d_walls, p->d_ghosts,
+                    p->d_state, p->d_case_first, p->d_maxbits, p->d_max_dt, p->d_fsum, p->d_fstate};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     delete p;
@@ -1331,11 +1445,11 @@ int ugks_step_plan_destroy(ugks_step_plan *p) {
 int ugks_step_run(ugks_step_plan *p, void *stream, int32_t nsteps) {
     if (!p || nsteps < 0) return ugks_rt::fail(-1, "step run: bad arguments");
     hipStream_t s = (hipStream_t)stream;
-    for (int k = 0; k < nsteps; k++) {
-        int rc = p->precision == UGKS_FP64 ? dispatch_npt<double>(p, s, nullptr) : dispatch_npt<float>(p, s, nullptr);
-        if (rc) return rc;
-    }
-    return 0;
+    if (nsteps == 0) return 0;
+    int rc = refresh(p, s);
+    for (int k = 0; k < nsteps && !rc; k++)
+        rc = p->precision == UGKS_FP64 ? dispatch_npt<double>(p, s, nullptr) : dispatch_npt<float>(p, s, nullptr);
+    return rc;
 }
 
 int ugks_step_run_timed(ugks_step_plan *p, void *stream, int32_t nsteps, float *kernel_ms) {
@@ -1348,7 +1462,7 @@ int ugks_step_run_timed(ugks_step_plan *p, void *stream, int32_t nsteps, float *
             return ugks_rt::fail(-1, "step run timed: event create");
         }
     for (int k = 0; k < kLaunches; k++) kernel_ms[k] = 0.0f;
-    int rc = 0;
+    int rc = nsteps > 0 ? refresh(p, s) : 0;
     for (int it = 0; it < nsteps && !rc; it++) {
         rc = p->precision == UGKS_FP64 ? dispatch_npt<double>(p, s, ev) : dispatch_npt<float>(p, s, ev);
         if (rc) break;
@@ -1421,6 +1535,9 @@ int ugks_step_run_graph(ugks_step_plan *p, void *stream, int32_t nsteps) {
         p->graph = g;
         p->graph_exec = x;
     }
+    if (nsteps == 0) return 0;
+    int rc = refresh(p, (hipStream_t)stream);
+    if (rc) return rc;
     for (int k = 0; k < nsteps; k++) {
         hipError_t e = hipGraphLaunch(p->graph_exec, (hipStream_t)stream);
         if (e != hipSuccess) return ugks_rt::hip_fail(e, "step graph: launch");

@@ -427,3 +427,120 @@ def test_find_dt_closed_form():
         h = 1.0 / 8
         freq = (h * u + h * v) / (h * h)
         assert abs(o.dt - 0.8 / freq.max()) < 1e-14 * o.dt
+
+
+# ------------------------------------------------------------ ghost fills --
+def _ghost_block(nc=12):
+    o = O.OracleSim(cases.cavity(8, ("NC", nc), "WENO5", 0.05, lid=0.1, n_cells_j=6), threads=2)
+    return o.blocks[0]
+
+
+def _call(name, b, *args):
+    ptrs = [C.c_void_p(a.ctypes.data) if isinstance(a, np.ndarray) else C.c_int(a) for a in args]
+    getattr(O.lib(), name)(C.byref(b.block.p), *ptrs)
+
+
+def test_edge_const_grad_is_exact_for_linear_data():
+    """edgeConstGrad (clGhostRoutines.c:967-1048) extends linear data exactly,
+    on every face; a wrong sign or distance in the extrapolation fails."""
+    b = _ghost_block()
+    NI, NJ, NV, G = b.consts["NI"], b.consts["NJ"], b.consts["NV"], b.consts["ghost"]
+    rng = np.random.default_rng(3)
+    a0, ai, aj = (rng.standard_normal((NV, 2)) for _ in range(3))
+    I, J = np.meshgrid(np.arange(NI), np.arange(NJ), indexing="ij")
+    exact = a0[None, None] + ai[None, None] * I[..., None, None] + aj[None, None] * J[..., None, None]
+    f = b["f"]
+    f[:] = 0.0
+    f[G:-G, G:-G] = exact[G:-G, G:-G]
+    for face in (bcd.NORTH, bcd.EAST, bcd.SOUTH, bcd.WEST):
+        _call("or_edgeConstGrad", b, f, face)
+    inner = slice(G, -G)
+    for sl in [(slice(0, G), inner), (slice(NI - G, NI), inner), (inner, slice(0, G)), (inner, slice(NJ - G, NJ))]:
+        assert np.max(np.abs(f[sl] - exact[sl])) < 1e-12
+
+
+def test_mirror_and_bounce_back_reverse_momentum():
+    """edgeMirror reverses the wall-normal momentum of the mirrored cells and
+    edgeBounceBack both components (clGhostRoutines.c:845-965)."""
+    b = _ghost_block()
+    c, (quad, _w, _mns, _mew, _md) = b.consts, b.tables
+    G, NV = c["ghost"], c["NV"]
+    rng = np.random.default_rng(4)
+    f = b["f"]
+    f[:] = rng.random(f.shape)
+    e = np.asarray(quad).reshape(NV, 2)
+
+    def mom(x):
+        return np.stack([np.sum(x[..., 0] * e[:, 0], -1), np.sum(x[..., 0] * e[:, 1], -1)], -1)
+
+    _call("or_edgeMirror", b, f, bcd.SOUTH)
+    _call("or_edgeMirror", b, f, bcd.WEST)
+    for k in range(G):
+        src, dst = mom(f[G:-G, 2 * G - 1 - k]), mom(f[G:-G, k])
+        assert np.allclose(dst[:, 0], src[:, 0], rtol=0, atol=1e-13) and np.allclose(dst[:, 1], -src[:, 1], rtol=0, atol=1e-13)
+        src, dst = mom(f[2 * G - 1 - k, G:-G]), mom(f[k, G:-G])
+        assert np.allclose(dst[:, 0], -src[:, 0], rtol=0, atol=1e-13) and np.allclose(dst[:, 1], src[:, 1], rtol=0, atol=1e-13)
+    _call("or_edgeBounceBack", b, f, bcd.NORTH)
+    NJ = c["NJ"]
+    for k in range(G):
+        src, dst = mom(f[G:-G, NJ - G - 1 - k]), mom(f[G:-G, NJ - G + k])
+        assert np.allclose(dst, -src, rtol=0, atol=1e-13)
+
+
+def test_constant_and_inflow_ghosts_carry_the_prescribed_state():
+    """edgeConstant fills the Maxwellian of wall_prop; edgeInflow (west face)
+    the one with the wall density and temperature and the adjacent cell's
+    velocity (clGhostRoutines.c:603-762): their discrete moments give back
+    that state to quadrature accuracy (41 x 41 Newton-Cotes points)."""
+    b = _ghost_block(40)
+    c, (quad, w, _a, _b2, _c2) = b.consts, b.tables
+    G, NV = c["ghost"], c["NV"]
+    e = np.asarray(quad).reshape(NV, 2)
+    wp = b["wall_prop"]
+    wp[:, bcd.EAST] = [1.3, 0.05, -0.02, 1.0 / 0.9]
+    b["Q"][:] = 0.0
+    b["macro"][:] = [1.0, 0.12, 0.03, 1.0]
+    wp[:, bcd.WEST] = [0.8, 0.0, 0.0, 1.0 / 1.1]
+    f = b["f"]
+    _call("or_edgeConstant", b, f, bcd.EAST, wp)
+    _call("or_edgeInflow", b, f, bcd.WEST, b["normal"], b["macro"], b["Q"], wp)
+
+    def prim(x):
+        rho = np.sum(x[..., 0], -1)
+        U = np.sum(x[..., 0] * e[:, 0], -1) / rho
+        V = np.sum(x[..., 0] * e[:, 1], -1) / rho
+        E = 0.5 * np.sum((e[:, 0] ** 2 + e[:, 1] ** 2) * x[..., 0] + x[..., 1], -1)
+        T = (E / rho - 0.5 * (U ** 2 + V ** 2)) * 2.0 / (2.0 + c["K"]) / 0.5   # (K+2)/2 R T per unit mass, R = 1/2
+        return rho, U, V, T
+
+    # composite Boole with h = 0.3 integrates these Maxwellians to ~2e-4
+    rho, U, V, T = prim(f[-1, G:-G])
+    assert np.allclose(rho, 1.3, rtol=1e-3) and np.allclose(U, 0.05, atol=1e-3)
+    assert np.allclose(V, -0.02, atol=1e-3) and np.allclose(T, 0.9, rtol=1e-3)
+    rho, U, V, T = prim(f[0, G:-G])
+    assert np.allclose(rho, 0.8, rtol=1e-3) and np.allclose(U, 0.12, atol=1e-3)
+    assert np.allclose(V, 0.03, atol=1e-3) and np.allclose(T, 1.1, rtol=1e-3)
+
+
+def test_residual_and_edge_mass_flow_closed_forms():
+    """getResidual is the squared change of the primary variables;
+    edgeMassFlow is (n.U) rho L with the outward normal (clUGKS.c:1259-1337)."""
+    b = _ghost_block()
+    c = b.consts
+    ni, nj = c["ni"], c["nj"]
+    rng = np.random.default_rng(5)
+    old = rng.random((ni, nj, 4))
+    b["residual"][:] = old
+    b["macro"][:] = old + 0.01 * rng.standard_normal((ni, nj, 4))
+    expect = (b["macro"] - old) ** 2
+    _call("or_getResidual", b, b["macro"], b["residual"])
+    assert np.allclose(b["residual"], expect, rtol=1e-15, atol=0)
+    b["macro"][:] = [1.2, 0.3, -0.1, 1.0]
+    mdot = b["bc_mdot"]
+    for face in range(4):
+        _call("or_edgeMassFlow", b, b["macro"], b["normal"], b["side_length"], mdot, face)
+    hx, hy = 1.0 / 8, 1.0 / 6
+    assert np.allclose(mdot[:ni, bcd.NORTH], -0.1 * 1.2 * hx, rtol=1e-13)
+    assert np.allclose(mdot[:ni, bcd.SOUTH], 0.1 * 1.2 * hx, rtol=1e-13)
+    assert np.allclose(mdot[:nj, bcd.EAST], 0.3 * 1.2 * hy, rtol=1e-13)
+    assert np.allclose(mdot[:nj, bcd.WEST], -0.3 * 1.2 * hy, rtol=1e-13)
