@@ -163,9 +163,10 @@ __device__ __forceinline__ WenoParts weno5_parts(double s0, double s1, double s2
     return WenoParts{p0 * c0 + p1 * c1 + p2 * c2, 0.1 * p0 + 0.6 * p1 + 0.3 * p2};
 }
 // the two slopes (f_L - s2) / dist of a (g, h) stencil
-__device__ __forceinline__ void weno5_pair(const P2<double> *s, double rdx, double &sgx, double &sgy) {
-    const WenoParts a = weno5_parts(s[0].x, s[1].x, s[2].x, s[3].x, s[4].x);
-    const WenoParts b = weno5_parts(s[0].y, s[1].y, s[2].y, s[3].y, s[4].y);
+__device__ __forceinline__ void weno5_pair(P2<double> s0, P2<double> s1, P2<double> s2, P2<double> s3, P2<double> s4,
+                                           double rdx, double &sgx, double &sgy) {
+    const WenoParts a = weno5_parts(s0.x, s1.x, s2.x, s3.x, s4.x);
+    const WenoParts b = weno5_parts(s0.y, s1.y, s2.y, s3.y, s4.y);
     // 1 / (den_a den_b) from the hardware estimate and two Newton steps; the
     // dens are >= 0.1 (1.2e-15)^4 and far below overflow, so is their product
     const double D = a.den * b.den;
@@ -195,9 +196,10 @@ __device__ __forceinline__ float weno5_f(float s0, float s1, float s2, float s3,
     const float c2 = (2.f / 3.f) * d3 - (1.f / 6.f) * d4;
     return (a0 * c0 + a1 * c1 + a2 * c2) * rcp_T(a0 + a1 + a2);
 }
-__device__ __forceinline__ void weno5_pair(const P2<float> *s, float rdx, float &sgx, float &sgy) {
-    sgx = weno5_f(s[0].x, s[1].x, s[2].x, s[3].x, s[4].x) * rdx;
-    sgy = weno5_f(s[0].y, s[1].y, s[2].y, s[3].y, s[4].y) * rdx;
+__device__ __forceinline__ void weno5_pair(P2<float> s0, P2<float> s1, P2<float> s2, P2<float> s3, P2<float> s4,
+                                           float rdx, float &sgx, float &sgy) {
+    sgx = weno5_f(s0.x, s1.x, s2.x, s3.x, s4.x) * rdx;
+    sgy = weno5_f(s0.y, s1.y, s2.y, s3.y, s4.y) * rdx;
 }
 template <typename T> __device__ __forceinline__ T vanleer_T(T s1, T s2) {
     T a1 = s1 < T(0) ? -s1 : s1, a2 = s2 < T(0) ? -s2 : s2;
@@ -263,42 +265,106 @@ template <typename T> __device__ __forceinline__ P2<T> sld(const Stencil<T> &st,
     return P2<T>{x.x, x.y};
 }
 
-// reconstructed face value and slope at velocity v (iFace); fl / fr, the
-// cells below / above the face, are read by the callers that need them
+// the stencil values of velocity v for upwind choice delta (5 cells for
+// WENO5, 3 otherwise), stencil index order; named members keep them in
+// registers across the load / compute split
+template <typename T> struct StenVals { P2<T> s0, s1, s2, s3, s4; };
 template <typename T>
-__device__ __forceinline__ void reconstruct(const Consts &C, const Stencil<T> &st, const FaceGeo &g, int v, int delta,
-                                            P2<T> &ff, P2<T> &sg, P2<T> *fl = nullptr, P2<T> *fr = nullptr) {
+__device__ __forceinline__ void stencil_load(const Consts &C, const Stencil<T> &st, int v, int delta, StenVals<T> &x) {
     const unsigned vb = (unsigned)v * (unsigned)sizeof(P2<T>);
     if (C.method == 1) {
-        P2<T> s[5];
-#pragma unroll
-        for (int k = 0; k < 5; k++) s[k] = sld<T>(st, delta ? k : 5 - k, vb);
-        const T dist = (T)(delta ? g.dist[1] : g.dist[0]), rdx = (T)(delta ? g.rdist[1] : g.rdist[0]);
-        weno5_pair(s, rdx, sg.x, sg.y);
-        ff.x = s[2].x + sg.x * dist;
-        ff.y = s[2].y + sg.y * dist;
-        if (fl) *fl = delta ? s[2] : s[3];
-        if (fr) *fr = delta ? s[3] : s[2];
+        x.s0 = sld<T>(st, delta ? 0 : 5, vb);
+        x.s1 = sld<T>(st, delta ? 1 : 4, vb);
+        x.s2 = sld<T>(st, delta ? 2 : 3, vb);
+        x.s3 = sld<T>(st, delta ? 3 : 2, vb);
+        x.s4 = sld<T>(st, delta ? 4 : 1, vb);
     } else {
-        P2<T> s[3];
-#pragma unroll
-        for (int k = 0; k < 3; k++) s[k] = sld<T>(st, delta ? k : 3 - k, vb);
+        x.s0 = sld<T>(st, delta ? 0 : 3, vb);
+        x.s1 = sld<T>(st, delta ? 1 : 2, vb);
+        x.s2 = sld<T>(st, delta ? 2 : 1, vb);
+        x.s3 = x.s4 = x.s2;
+    }
+}
+
+// reconstructed face value and slope (iFace) from the stencil values; fl /
+// fr, the cells below / above the face, on request
+template <typename T>
+__device__ __forceinline__ void reconstruct_from(const Consts &C, const FaceGeo &g, int delta, const StenVals<T> &x,
+                                                 P2<T> &ff, P2<T> &sg, P2<T> *fl = nullptr, P2<T> *fr = nullptr) {
+    if (C.method == 1) {
+        const T dist = (T)(delta ? g.dist[1] : g.dist[0]), rdx = (T)(delta ? g.rdist[1] : g.rdist[0]);
+        weno5_pair(x.s0, x.s1, x.s2, x.s3, x.s4, rdx, sg.x, sg.y);
+        ff.x = x.s2.x + sg.x * dist;
+        ff.y = x.s2.y + sg.y * dist;
+        if (fl) *fl = P2<T>{delta ? x.s2.x : x.s3.x, delta ? x.s2.y : x.s3.y};
+        if (fr) *fr = P2<T>{delta ? x.s3.x : x.s2.x, delta ? x.s3.y : x.s2.y};
+    } else {
         const T dist = (T)(delta ? g.dist[1] : g.dist[0]);
         if (C.method == 0) {
             const T r1 = (T)(delta ? g.rd1[1] : g.rd1[0]);
-            sg.x = vanleer_T<T>((s[1].x - s[0].x) * r1, (s[2].x - s[1].x) * r1);
-            sg.y = vanleer_T<T>((s[1].y - s[0].y) * r1, (s[2].y - s[1].y) * r1);
+            sg.x = vanleer_T<T>((x.s1.x - x.s0.x) * r1, (x.s2.x - x.s1.x) * r1);
+            sg.y = vanleer_T<T>((x.s1.y - x.s0.y) * r1, (x.s2.y - x.s1.y) * r1);
         } else {
             const T r = (T)(delta ? g.rdist[1] : g.rdist[0]);
-            sg.x = (s[1].x + T(0.5) * minmod_T<T>(s[2].x - s[1].x, s[1].x - s[0].x) - s[1].x) * r;
-            sg.y = (s[1].y + T(0.5) * minmod_T<T>(s[2].y - s[1].y, s[1].y - s[0].y) - s[1].y) * r;
+            sg.x = (x.s1.x + T(0.5) * minmod_T<T>(x.s2.x - x.s1.x, x.s1.x - x.s0.x) - x.s1.x) * r;
+            sg.y = (x.s1.y + T(0.5) * minmod_T<T>(x.s2.y - x.s1.y, x.s1.y - x.s0.y) - x.s1.y) * r;
         }
-        ff.x = s[1].x + sg.x * dist;
-        ff.y = s[1].y + sg.y * dist;
-        if (fl) *fl = delta ? s[1] : s[2];
-        if (fr) *fr = delta ? s[2] : s[1];
+        ff.x = x.s1.x + sg.x * dist;
+        ff.y = x.s1.y + sg.y * dist;
+        if (fl) *fl = P2<T>{delta ? x.s1.x : x.s2.x, delta ? x.s1.y : x.s2.y};
+        if (fr) *fr = P2<T>{delta ? x.s2.x : x.s1.x, delta ? x.s2.y : x.s1.y};
     }
 }
+
+template <typename T>
+__device__ __forceinline__ void reconstruct(const Consts &C, const Stencil<T> &st, const FaceGeo &g, int v, int delta,
+                                            P2<T> &ff, P2<T> &sg, P2<T> *fl = nullptr, P2<T> *fr = nullptr) {
+    StenVals<T> x;
+    stencil_load<T>(C, st, v, delta, x);
+    reconstruct_from<T>(C, g, delta, x, ff, sg, fl, fr);
+}
+
+// Velocity loop of a wavefront-per-face pass, v = lane, lane + 64, ...:
+// the stencil of the next velocity is loaded before the current one is
+// used, so the loads overlap the arithmetic.
+#ifndef UGKS_PREFETCH
+#define UGKS_PREFETCH 0
+#endif
+template <typename T, typename Body>
+__device__ __forceinline__ void face_velocity_loop(const Consts &C, const Stencil<T> &st, d2 n, int lane, Body body) {
+#if UGKS_PREFETCH
+    int v = lane;
+    if (v >= C.NV) return;
+    d2 uv = face_velocity(C, v, n);
+    int delta = upwind(uv.x);
+    StenVals<T> cur;
+    stencil_load<T>(C, st, v, delta, cur);
+    while (true) {
+        const int vn = v + 64;
+        const bool more = vn < C.NV;
+        d2 uvn = uv;
+        int dn = delta;
+        StenVals<T> nxt;
+        if (more) {
+            uvn = face_velocity(C, vn, n);
+            dn = upwind(uvn.x);
+            stencil_load<T>(C, st, vn, dn, nxt);
+        }
+        body(v, uv, delta, cur);
+        if (!more) break;
+        v = vn; uv = uvn; delta = dn; cur = nxt;
+    }
+#else
+    for (int v = lane; v < C.NV; v += 64) {
+        const d2 uv = face_velocity(C, v, n);
+        const int delta = upwind(uv.x);
+        StenVals<T> cur;
+        stencil_load<T>(C, st, v, delta, cur);
+        body(v, uv, delta, cur);
+    }
+#endif
+}
+
 // the cells below (rel R-1) and above (rel R, the owner) the face
 template <typename T>
 __device__ __forceinline__ void face_cells(const Consts &C, const Stencil<T> &st, int v, P2<T> &fl, P2<T> &fr) {
@@ -353,10 +419,9 @@ FACE_KERNEL k_faceA(const FBlock *__restrict__ blocks, const long long *__restri
     double a[kNS];
 #pragma unroll
     for (int q = 0; q < kNS; q++) a[q] = 0.0;
-    for (int v = lane; v < C.NV; v += 64) {
-        const d2 uv = face_velocity(C, v, n);
+    face_velocity_loop<T>(C, stn, n, lane, [&](int v, d2 uv, int delta, const StenVals<T> &sv) __attribute__((always_inline)) {
         P2<T> ff, sg, fl, fr;
-        reconstruct<T>(C, stn, g, v, upwind(uv.x), ff, sg, &fl, &fr);
+        reconstruct_from<T>(C, g, delta, sv, ff, sg, &fl, &fr);
         const double u = uv.x, w = uv.y, gx = (double)ff.x, hx = (double)ff.y;
         const double ug = u * gx, wg = w * gx, uug = u * ug, wwg = w * wg;
         a[0] += gx; a[1] += ug; a[2] += wg;
@@ -370,7 +435,7 @@ FACE_KERNEL k_faceA(const FBlock *__restrict__ blocks, const long long *__restri
             a[13] += 0.5 * (cx * t);
             a[14] += 0.5 * (cy * t);
         }
-    }
+    });
 #pragma unroll
     for (int q = 0; q < kNS; q++) a[q] = wave_sum(a[q]);
     if (lane == 0) {
@@ -491,11 +556,9 @@ FACE_KERNEL k_faceC(const FBlock *__restrict__ blocks, const long long *__restri
     // tensor indices of v = lane + 64 k, advanced without integer division
     int iu = 0, jv = 0, diu = 0, djv = 0;
     if (sep) { iu = lane / C.nvt; jv = lane - iu * C.nvt; diu = 64 / C.nvt; djv = 64 - diu * C.nvt; }
-    for (int v = lane; v < C.NV; v += 64) {
-        const d2 uv = face_velocity(C, v, n);
-        const int delta = upwind(uv.x);
+    face_velocity_loop<T>(C, stn, n, lane, [&](int v, d2 uv, int delta, const StenVals<T> &stv) __attribute__((always_inline)) {
         P2<T> ff, sg;
-        reconstruct<T>(C, stn, g, v, delta, ff, sg);
+        reconstruct_from<T>(C, g, delta, stv, ff, sg);
         const double ux = uv.x, uy = uv.y, cx = ux - prim.s1, cy = uy - prim.s2, c2 = cx * cx + cy * cy;
         const double E = sep ? Ei[iu] * Ej[jv] : exp(-prim.s3 * c2);
         if (sep) {
@@ -521,7 +584,7 @@ FACE_KERNEL k_faceC(const FBlock *__restrict__ blocks, const long long *__restri
         mf[1] += ux * X;
         mf[2] += uy * X;
         mf[3] += 0.5 * (uu * X + ux * (m0 * shh * (hk * f0g) + th));
-    }
+    });
 #pragma unroll
     for (int q = 0; q < 4; q++) mf[q] = wave_sum(mf[q]);
     if (lane == 0) {
