@@ -213,10 +213,12 @@ typedef struct ugks_bc_desc {
     int32_t type, other_block, other_face, orientation, flip, reflect_type;
 } ugks_bc_desc;
 
-/* Time-step state (the gdata fields of ugks_sim.py:285-311, :491). */
+/* Time-step state (the gdata fields of ugks_sim.py:285-311, :491); overshoot
+ * reports a first step whose re-evaluated dt differed (ugks_sim.py:443-445);
+ * force_time holds dt fixed (UGKSim.force_time, ugks_sim.py:292, :673-675). */
 typedef struct ugks_dt_desc {
     double dt, time, CFL, delta_dt, suggest_dt, Kn_eff;
-    int32_t step, overshoot;
+    int32_t step, overshoot, force_time, pad;
 } ugks_dt_desc;
 
 typedef struct ugks_step_plan ugks_step_plan;
@@ -248,6 +250,8 @@ int ugks_step_state(ugks_step_plan *plan, void *stream, ugks_dt_desc *out, doubl
 int ugks_step_states(ugks_step_plan *plan, void *stream, ugks_dt_desc *out, double *max_dt);
 int ugks_step_set_dt(ugks_step_plan *plan, void *stream, double dt);
 int ugks_step_set_dt_case(ugks_step_plan *plan, void *stream, int32_t case_index, double dt);
+/* set dt and hold it for the remaining steps (the last step that lands on max_time) */
+int ugks_step_force_dt_case(ugks_step_plan *plan, void *stream, int32_t case_index, double dt);
 /* plain device-to-device copy kernel of `bytes` (a multiple of 16): the
  * reference stream for bandwidth and byte-counter calibration */
 int ugks_copy_kernel(void *stream, const void *src, void *dst, int64_t bytes);

@@ -223,15 +223,23 @@ class UGKSBlock:
         if self.get_flag():
             return True
         if get_residual:
-            self.get_BC_mdot()
-            p.getResidual(s, self.macro_D, self.residual_D)
-            res = self.residual_D.download(s)
-            macro = self.macro_D.download(s)
-            sum_res = np.sum(np.sum(res, axis=0), axis=0)
-            sum_avg = np.sum(np.sum(np.abs(macro), axis=0), axis=0)
-            self.residual = np.sqrt(self.ni * self.nj * sum_res) / (sum_avg + np.finfo(float).tiny)
-            self.residual[self.residual <= 0.0] = 1.0
+            self.compute_residual()
         return False
+
+    def compute_residual(self):
+        """The residual part of ugks_block.py:1303-1327: boundary mass flow,
+        then the normalised RMS change of (rho, U, V, lambda), from residual_D
+        holding the previous primary variables (updateMacro stores them)."""
+        s, p = self.stream, self.prg
+        self.get_BC_mdot()
+        p.getResidual(s, self.macro_D, self.residual_D)
+        res = self.residual_D.download(s)
+        macro = self.macro_D.download(s)
+        sum_res = np.sum(np.sum(res, axis=0), axis=0)
+        sum_avg = np.sum(np.sum(np.abs(macro), axis=0), axis=0)
+        self.residual = np.sqrt(self.ni * self.nj * sum_res) / (sum_avg + np.finfo(float).tiny)
+        self.residual[self.residual <= 0.0] = 1.0
+        return self.residual
 
     def get_BC_mdot(self):
         """ugks_block.py:889-924"""

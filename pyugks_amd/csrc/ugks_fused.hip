@@ -83,7 +83,7 @@ struct FBlock {
 
 struct DtState {
     double dt, time, CFL, delta_dt, suggest_dt, Kn;
-    int step, overshoot;
+    int step, overshoot, force_time, pad;
 };
 
 // Workgroups are dealt to the 8 XCDs round robin (workgroup id % 8), and
@@ -1040,6 +1040,11 @@ __global__ void k_dtfin(DtState *dss, unsigned long long *maxbits, double *max_d
         M = fmin(M, md);
         maxbits[b] = 0ull;
     }
+    if (ds->force_time) {       // dt held at the value that lands on max_time (ugks_sim.py:292, :673-675)
+        ds->time += ds->dt;
+        ds->step += 1;
+        return;
+    }
     const double lim = 1.0 + ds->delta_dt;
     double d1 = ds->dt;
     if (d1 != M) d1 = (M > lim * d1) ? lim * d1 : M;
@@ -1467,7 +1472,8 @@ int ugks_step_plan_create_batch(ugks_block *const *blocks, const ugks_block_buff
     if (e != hipSuccess) { ugks_step_plan_destroy(p); return ugks_rt::hip_fail(e, "step plan: hipMalloc"); }
     std::vector<DtState> st(ncases);
     for (int c = 0; c < ncases; c++)
-        st[c] = DtState{dtd[c].dt, dtd[c].time, dtd[c].CFL, dtd[c].delta_dt, dtd[c].suggest_dt, dtd[c].Kn_eff, dtd[c].step, 0};
+        st[c] = DtState{dtd[c].dt, dtd[c].time, dtd[c].CFL, dtd[c].delta_dt, dtd[c].suggest_dt, dtd[c].Kn_eff,
+                        dtd[c].step, 0, dtd[c].force_time, 0};
     std::vector<unsigned long long> zeros(nblocks, 0ull);
     if (e == hipSuccess)
         for (int b = 0; b < nblocks; b++) p->host_blocks[b].cmom = p->d_cmom + all_off[b];
@@ -1552,7 +1558,7 @@ int ugks_step_states(ugks_step_plan *p, void *stream, ugks_dt_desc *out, double 
     for (int c = 0; c < p->ncases; c++) {
         out[c].dt = st[c].dt; out[c].time = st[c].time; out[c].CFL = st[c].CFL; out[c].delta_dt = st[c].delta_dt;
         out[c].suggest_dt = st[c].suggest_dt; out[c].Kn_eff = st[c].Kn; out[c].step = st[c].step;
-        out[c].overshoot = st[c].overshoot;
+        out[c].overshoot = st[c].overshoot; out[c].force_time = st[c].force_time; out[c].pad = 0;
     }
     return 0;
 }
@@ -1577,6 +1583,16 @@ int ugks_step_set_dt_case(ugks_step_plan *p, void *stream, int32_t case_index, d
 }
 
 int ugks_step_set_dt(ugks_step_plan *p, void *stream, double dt) { return ugks_step_set_dt_case(p, stream, 0, dt); }
+
+int ugks_step_force_dt_case(ugks_step_plan *p, void *stream, int32_t case_index, double dt) {
+    if (!p || case_index < 0 || case_index >= p->ncases) return ugks_rt::fail(-1, "force dt: bad arguments");
+    DtState *d = p->d_state + case_index;
+    const int one = 1;
+    hipError_t e = hipMemcpyAsync(&d->dt, &dt, sizeof(double), hipMemcpyHostToDevice, (hipStream_t)stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(&d->force_time, &one, sizeof(int), hipMemcpyHostToDevice, (hipStream_t)stream);
+    if (e == hipSuccess) e = hipStreamSynchronize((hipStream_t)stream);
+    return e == hipSuccess ? 0 : ugks_rt::hip_fail(e, "force dt");
+}
 
 int ugks_step_run_graph(ugks_step_plan *p, void *stream, int32_t nsteps) {
     if (!p || nsteps < 0) return ugks_rt::fail(-1, "step run graph: bad arguments");

@@ -97,6 +97,23 @@ class BlockDef:
 
 
 @dataclass
+class ResidualOptions:
+    """ugks_data.py:74-102: residual monitoring and the stop criteria built on it."""
+    global_residual: object = 1e50
+    residual_history: list = field(default_factory=list)
+    get_residual: bool = False
+    residual_count: int = -1
+    min_residual: float = 1e-50
+    min_slope: float = 1e-3
+    slope_start: int = 10
+    residual_start: int = 10
+    res_drop: float = float("inf")
+    non_linear_output: bool = False
+    non_linear_output_limit: float = 1e5
+    non_linear_dt: bool = False
+
+
+@dataclass
 class UGKSData:
     """The gdata slots the time step reads, with the reference defaults
     (ugks_data.py:146-235)."""
@@ -110,6 +127,12 @@ class UGKSData:
     max_time: float = 1.0e-3
     max_step: int = 1
     dt_update_count: int = 1
+    # step-loop controls (ugks_data.py:158-178)
+    print_count: int = 20
+    clock_time_stop: tuple = (1e6, 0)
+    exit: bool = False
+    runtime_conf: str = ""
+    residual_options: ResidualOptions = field(default_factory=ResidualOptions)
     L_ref: float = 1.0
     D_ref: float = 1.0
     T_ref: float = 273.0

@@ -73,7 +73,7 @@ class StepPlan:
         for ci, sim in enumerate(self.sims):
             g = sim.gdata
             dts[ci] = L.DtDesc(g.dt, g.time, g.CFL, g.delta_dt, g.suggest_dt if g.suggest_dt else 0.0, g.Kn_eff,
-                               int(sim.step), 0)
+                               int(sim.step), 0, int(bool(getattr(sim, "force_time", False))), 0)
         h = C.c_void_p()
         L.check(lib.ugks_step_plan_create_batch(handles, bufs, bcs, nb, case_of, len(self.sims),
                                                 L.UGKS_MIXED if mixed else L.UGKS_FP64, dts, C.byref(h)),
@@ -114,6 +114,12 @@ class StepPlan:
 
     def state(self, update_sim: bool = True):
         return self.states(update_sim)[0]
+
+    def set_dt(self, dt: float, case: int = 0, hold: bool = True):
+        """Set one case's time step; with ``hold`` it stays fixed for the
+        remaining steps (the shortened last step onto max_time)."""
+        fn = L.load().ugks_step_force_dt_case if hold else L.load().ugks_step_set_dt_case
+        L.check(fn(self.handle, self.stream, int(case), float(dt)), "set dt")
 
     def f64(self, block):
         """f of one block as FP64 (converted from FP32 for the mixed step)."""

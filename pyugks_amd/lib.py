@@ -72,6 +72,7 @@ FUSED = {
     "ugks_step_states": [P, P, P, P],
     "ugks_step_set_dt": [P, P, D],
     "ugks_step_set_dt_case": [P, P, I, D],
+    "ugks_step_force_dt_case": [P, P, I, D],
     "ugks_copy_kernel": [P, P, P, C.c_int64],
     "ugks_f64_to_f32": [P, P, P, C.c_int64],
     "ugks_f32_to_f64": [P, P, P, C.c_int64],
@@ -94,7 +95,7 @@ class BCDesc(C.Structure):
 
 class DtDesc(C.Structure):
     _fields_ = [("dt", D), ("time", D), ("CFL", D), ("delta_dt", D), ("suggest_dt", D), ("Kn_eff", D),
-                ("step", I), ("overshoot", I)]
+                ("step", I), ("overshoot", I), ("force_time", I), ("pad", I)]
 
 RUNTIME = {
     "ugks_block_create": [P, C.POINTER(P)],

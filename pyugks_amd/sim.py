@@ -1,8 +1,11 @@
 """Host counterpart of ugks_sim.UGKSim: block set-up and the time loop.
 
 Follows ugks_sim.py:52-115 (initialisation order), :274-321 (time step and
-BC refresh), :407-501 (one_step) and :503-680 (the step loop, without file
-output, plotting, the residual-slope analysis or the runtime .conf re-read).
+BC refresh), :407-501 (one_step, including the residual and mass monitors)
+and :503-680 (the step loop: residual cadence, stop criteria, clock limit,
+the runtime control file and the final-time clamp).  Not rebuilt: file
+output and restart, plotting, and the exec of user scripts (the runtime
+control file is read as plain ``name = literal`` assignments instead).
 """
 from __future__ import annotations
 
@@ -92,36 +95,183 @@ class UGKSim:
             self.calcTimeStep()
             self.updateTimeStep()
         if get_res:
-            res = sum(b.residual for b in self.blocks) / len(self.blocks)
-            g.global_residual = res
+            self._residual_monitor()
         g.time += g.dt
         self.step += 1
         g.step = self.step
         self.updateAllBC()
         return True, False
 
-    def run(self, nsteps=None, max_time=None):
-        """The step loop of ugks_sim.py:542-678 (stop on NaN, step or time limit)."""
+    def _residual_monitor(self):
+        """ugks_sim.py:462-488: mean block residual, plus |boundary mass flow|
+        and the relative change of the total (bulk + adsorbed) mass for open
+        domains."""
         g = self.gdata
-        nsteps = g.max_step if nsteps is None else nsteps
+        res = sum(np.asarray(b.residual, dtype=np.float64) for b in self.blocks) / len(self.blocks)
+        if self.is_open:
+            mdot = sum(b.bc_mdot for b in self.blocks)
+            res = np.append(res, [np.nan if mdot == 0.0 else abs(mdot)])
+            bulk, ads = self.total_mass()
+            m0 = getattr(self, "initial_mass_total", bulk + ads)
+            dm = abs((m0 - (bulk + ads)) / m0)
+            res = np.append(res, [dm if dm > 0.0 else np.nan])
+        g.residual_options.global_residual = res
+        g.residual_options.residual_history.append(res)
+        g.global_residual = res
+        return res
+
+    def analyse_residual(self):
+        """ugks_sim.py:722-777 without the plot: the slope of each residual
+        component on a log-log scale, from the resampled history after a
+        second-order Butterworth low-pass (Wn = 1 / n^0.8) run forwards and
+        backwards; inf until slope_start."""
+        from scipy import signal
+        res = self.gdata.residual_options
+        if self.step < res.slope_start:
+            return np.inf
+        res_x = np.log10(np.asarray(self.time_history_residual_N, dtype=np.float64))
+        res_y = np.log10(np.asarray(self.time_history_residual, dtype=np.float64))
+        dx = np.min(np.diff(res_x))
+        sample_x = np.arange(res_x[0], res_x[-1] + dx, dx)
+        n = len(res_x)
+        A, B = signal.butter(2, 1.0 / n ** 0.8, output="ba")
+        slopes = []
+        for i in range(res_y.shape[1]):
+            ry = res_y[:, i]
+            mask = np.isfinite(ry)
+            filt = signal.filtfilt(A, B, np.interp(sample_x, res_x[mask], ry[mask]))
+            slopes.append(np.abs(np.diff(filt) / dx)[-1])
+        return slopes
+
+    def read_runtime_conf(self):
+        """The per-step re-read of gdata.runtime_conf (ugks_data.py:373-380),
+        restricted to ``name = literal`` lines for the step-loop controls."""
+        import ast
+        g = self.gdata
+        path = g.runtime_conf
+        if not path:
+            return
+        try:
+            with open(path) as fh:
+                text = fh.read()
+        except OSError:
+            return
+        allowed = {"exit", "max_time", "max_step", "print_count", "CFL", "suggest_dt", "delta_dt",
+                   "dt_update_count", "clock_time_stop"}
+        for line in text.splitlines():
+            line = line.split("#", 1)[0].strip()
+            if "=" not in line:
+                continue
+            key, val = (x.strip() for x in line.split("=", 1))
+            if key.startswith("gdata."):
+                key = key[6:]
+            if key in allowed:
+                setattr(g, key, ast.literal_eval(val))
+
+    def _plan_step(self, plan, get_res):
+        """One step through a fused plan (dt re-evaluated every step)."""
+        g = self.gdata
+        plan.run(1)
+        d = plan.state()
+        if d.overshoot:
+            raise RuntimeError("Overshot on first iteration, decrease time step")
+        if plan.flag():
+            self.last_error = "error flag"
+            return True
+        if get_res:
+            for b in self.blocks:
+                b.compute_residual()
+            self._residual_monitor()
+        return False
+
+    def run(self, nsteps=None, max_time=None, plan=None, log=print):
+        """The step loop of ugks_sim.py:503-678.
+
+        Stops, with ``self.exit_reason`` set, on: gdata.exit (runtime control
+        file), a NaN, the time limit, the clock limit, a residual slope below
+        min_slope twice in a row, the residual drop reaching res_drop, every
+        residual component below min_residual, or the step count (nsteps
+        more steps, else gdata.max_step).  The last step is shortened to land
+        on max_time exactly (force_time).  With ``plan`` (a fused StepPlan
+        over this simulation) the steps run on the device; the time step is
+        then re-evaluated every step (dt_update_count = 1).
+        """
+        g = self.gdata
+        res = g.residual_options
+        max_step = self.step + nsteps if nsteps is not None else g.max_step
         t_end = g.max_time if max_time is None else max_time
+        if plan is not None and g.dt_update_count != 1:
+            raise ValueError("a fused plan re-evaluates dt every step (dt_update_count must be 1)")
+        clock = g.clock_time_stop[0] * 3600.0 + g.clock_time_stop[1] * 60.0
+        mag = 1
+        below = 0
+        first_res = True
+        initial_res = None
+        bulk, ads = self.total_mass()
+        self.initial_mass_total = bulk + ads
+        self.time_history_residual, self.time_history_residual_N = [], []
+        self.exit_reason = None
         t0 = time.time()
-        target = self.step + nsteps
-        while self.step < target:
+        while self.step < max_step:
+            if res.get_residual and res.non_linear_output and (self.step % mag == 0) \
+                    and (self.step < res.non_linear_output_limit):
+                mag = int(10 ** np.floor(np.log10(self.step + 1)))
+                res.residual_count = mag
+                if res.non_linear_dt:
+                    g.dt_update_count = mag
+            get_res = bool(res.get_residual and ((self.step + 1) % res.residual_count == 0))
             if g.dt_update_count < 0:
                 get_dt = self.step == -g.dt_update_count
             else:
                 get_dt = not (self.step + 1) % g.dt_update_count
-            _, err = self.one_step(get_dt)
-            if err:
-                return False
-            if g.time >= t_end:
+            if plan is None:
+                _, err = self.one_step(get_dt, get_res)
+            else:
+                err = self._plan_step(plan, get_res)
+            self.read_runtime_conf()
+            if g.exit:
+                self.exit_reason = "interrupted"
                 break
+            if err:
+                self.exit_reason = "NaN encountered"
+                break
+            if g.time >= t_end:
+                self.exit_reason = "time limit reached"
+                break
+            if (time.time() - t0) >= clock:
+                self.exit_reason = "clock time limit reached"
+                break
+            if get_res:
+                self.time_history_residual.append(res.global_residual)
+                self.time_history_residual_N.append(self.step)
+                slope = self.analyse_residual()
+                if self.step >= res.slope_start:
+                    below = below + 1 if np.all(np.asarray(slope) < res.min_slope) else 0
+                    if below >= 2:
+                        self.exit_reason = "residual stabilised"
+                        break
+                if self.step >= res.residual_start:
+                    if first_res:
+                        initial_res = res.global_residual
+                        first_res = False
+                    elif np.all(initial_res / res.global_residual >= res.res_drop):
+                        self.exit_reason = "residual reduction limit reached"
+                        break
+                    if np.all(res.global_residual <= res.min_residual):
+                        self.exit_reason = "minimum residual reached"
+                        break
+            if log and g.print_count > 0 and (not self.step % g.print_count or get_res):
+                log("step %d, t = %0.5f" % (self.step, g.time))
             if (t_end - g.time) < g.dt:
                 g.dt = t_end - g.time
                 self.force_time = True
+                if plan is not None:
+                    plan.set_dt(g.dt)
+        if self.exit_reason is None:
+            self.exit_reason = "step count reached"
         self.wall_time = time.time() - t0
-        return True
+        return self.exit_reason in ("step count reached", "time limit reached", "residual stabilised",
+                                    "residual reduction limit reached", "minimum residual reached")
 
     def total_mass(self):
         """ugks_sim.py:1119-1131"""

@@ -36,7 +36,7 @@ def test_descriptor_layouts():
     assert C.sizeof(lib.BlockDesc) == expect
     assert C.sizeof(lib.BlockBuffers) == 21 * 8
     assert C.sizeof(lib.BCDesc) == 6 * 4
-    assert C.sizeof(lib.DtDesc) == 6 * 8 + 2 * 4
+    assert C.sizeof(lib.DtDesc) == 6 * 8 + 4 * 4
 
 
 def test_missing_library_fails_loudly(tmp_path, monkeypatch):
@@ -104,3 +104,47 @@ def test_batch_cases_are_seeded_and_uneven():
     assert len(set(sides)) > 3
     kn = [x.Kn_eff for x in a]
     assert min(kn) >= 1e-3 and max(kn) <= 1.0
+
+
+def _bare_sim(tmp_path=None):
+    from pyugks_amd.data import UGKSData
+    from pyugks_amd.sim import UGKSim
+    s = UGKSim.__new__(UGKSim)
+    s.gdata = UGKSData()
+    s.step = 0
+    return s
+
+
+def test_runtime_control_file_sets_only_loop_controls(tmp_path):
+    """The per-step runtime file (ugks_data.py:373-380) is read as literal
+    assignments; anything else is ignored, nothing is executed."""
+    s = _bare_sim()
+    conf = tmp_path / "run.conf"
+    conf.write_text("gdata.max_time = 2.5\nexit = True\nCFL = 0.5  # comment\n"
+                    "import os\nR = 3.0\nmax_step = __import__('os').getpid()\n")
+    s.gdata.runtime_conf = str(conf)
+    with pytest.raises(ValueError):
+        s.read_runtime_conf()            # a non-literal value is refused, not evaluated
+    conf.write_text("gdata.max_time = 2.5\nexit = True\nCFL = 0.5  # comment\nimport os\nR = 3.0\n")
+    s.read_runtime_conf()
+    assert s.gdata.max_time == 2.5 and s.gdata.exit is True and s.gdata.CFL == 0.5
+    assert s.gdata.R == 287.0
+
+
+def test_residual_slope_orders_power_laws():
+    """analyse_residual (ugks_sim.py:722-777): the filtered log-log slope of
+    the residual history.  The reference's short filtfilt padding biases the
+    end-point estimate low, so what is checked is what its stop criterion
+    relies on: a flat history gives slope 0 (it stops), and a steeper decay
+    gives a larger slope, component by component."""
+    s = _bare_sim()
+    s.gdata.residual_options.slope_start = 10
+    N = np.arange(1, 400, 3)
+    s.time_history_residual_N = list(N)
+    s.time_history_residual = [np.array([n ** -1.5, 2 * n ** -0.5, 0.3]) for n in N]
+    s.step = int(N[-1])
+    slopes = s.analyse_residual()
+    assert slopes[0] > slopes[1] > 10 * s.gdata.residual_options.min_slope
+    assert slopes[2] < 1e-12
+    s.step = 5
+    assert s.analyse_residual() == np.inf

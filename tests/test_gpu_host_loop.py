@@ -1,0 +1,77 @@
+"""The host step loop (ugks_sim.py:503-678) on the GPU: the last step is
+shortened onto max_time, the residual monitor and its stop criteria, the
+open-domain mass monitor, and the same loop driven through a fused plan."""
+import copy
+
+import numpy as np
+import pytest
+
+from helpers import rel_err
+from pyugks_amd import cases
+
+pytestmark = pytest.mark.gpu
+
+
+def _cavity():
+    return cases.cavity(8, ("NC", 8), "WENO5", 0.05, lid=0.2, n_cells_j=6)
+
+
+@pytest.mark.parametrize("fused", [False, True])
+def test_run_lands_on_max_time(hip, fused):
+    from pyugks_amd.fused import StepPlan
+    from pyugks_amd.sim import UGKSim
+    print(f"[gpu] host loop max_time fused={fused}", flush=True)
+    sim = UGKSim(copy.deepcopy(_cavity()))
+    t_end = 5.5 * sim.gdata.dt
+    plan = StepPlan(sim, "fp64") if fused else None
+    assert sim.run(nsteps=100, max_time=t_end, plan=plan, log=None)
+    assert sim.exit_reason == "time limit reached"
+    assert sim.force_time
+    assert abs(sim.gdata.time - t_end) <= 1e-15 * t_end
+    assert sim.step in (6, 7)
+
+
+def test_fused_and_per_kernel_loops_agree(hip):
+    from pyugks_amd.fused import StepPlan
+    from pyugks_amd.sim import UGKSim
+    a, b = UGKSim(copy.deepcopy(_cavity())), UGKSim(copy.deepcopy(_cavity()))
+    t_end = 4.25 * a.gdata.dt
+    a.run(nsteps=50, max_time=t_end, log=None)
+    b.run(nsteps=50, max_time=t_end, plan=StepPlan(b, "fp64"), log=None)
+    assert a.step == b.step and abs(a.gdata.time - b.gdata.time) <= 1e-14 * t_end
+    assert rel_err(b.blocks[0].macro(), a.blocks[0].macro()) <= 1e-12
+    assert rel_err(b.blocks[0].f(), a.blocks[0].f()) <= 1e-12
+
+
+@pytest.mark.parametrize("fused", [False, True])
+def test_residual_monitor_and_minimum_residual_stop(hip, fused):
+    from pyugks_amd.fused import StepPlan
+    from pyugks_amd.sim import UGKSim
+    sim = UGKSim(copy.deepcopy(_cavity()))
+    r = sim.gdata.residual_options
+    r.get_residual, r.residual_count, r.residual_start, r.slope_start = True, 2, 6, 100
+    r.min_residual = 1e-300
+    assert sim.run(nsteps=8, plan=StepPlan(sim, "fp64") if fused else None, log=None)
+    assert sim.exit_reason == "step count reached"
+    assert sim.time_history_residual_N == [1, 3, 5, 7]
+    hist = np.array(sim.time_history_residual)
+    assert hist.shape == (4, 4) and np.all(hist > 0) and np.all(np.isfinite(hist))
+    # the lid drives the flow: the density residual falls from its first value
+    r.min_residual = 1.0
+    sim.run(nsteps=20, plan=StepPlan(sim, "fp64") if fused else None, log=None)
+    assert sim.exit_reason == "minimum residual reached"
+
+
+def test_open_domain_mass_monitor(hip):
+    from pyugks_amd.sim import UGKSim
+    g = cases.cfg5_channel(widths=[4, 4], height=4, nc=8)
+    sim = UGKSim(copy.deepcopy(g))
+    assert sim.is_open
+    r = sim.gdata.residual_options
+    r.get_residual, r.residual_count, r.slope_start, r.residual_start = True, 1, 100, 100
+    sim.run(nsteps=3, log=None)
+    res = sim.gdata.residual_options.global_residual
+    assert res.shape == (6,)
+    assert res[4] > 0                       # |boundary mass flow|: the inflow carries mass in
+    bulk, ads = sim.total_mass()
+    assert abs(res[5] - abs(sim.initial_mass_total - (bulk + ads)) / sim.initial_mass_total) < 1e-14
