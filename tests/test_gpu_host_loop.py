@@ -53,7 +53,7 @@ def test_residual_monitor_and_minimum_residual_stop(hip, fused):
     r.min_residual = 1e-300
     assert sim.run(nsteps=8, plan=StepPlan(sim, "fp64") if fused else None, log=None)
     assert sim.exit_reason == "step count reached"
-    assert sim.time_history_residual_N == [1, 3, 5, 7]
+    assert sim.time_history_residual_N == [2, 4, 6, 8]      # recorded after the step (step + 1) % 2 == 0
     hist = np.array(sim.time_history_residual)
     assert hist.shape == (4, 4) and np.all(hist > 0) and np.all(np.isfinite(hist))
     # every normalised residual is far below 1: the next run stops on the first check
