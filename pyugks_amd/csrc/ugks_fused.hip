@@ -875,20 +875,26 @@ __global__ void __launch_bounds__(kT) k_cellmom(const FBlock *__restrict__ block
 }
 
 // ============================================================ k_update ====
-// one wavefront per cell: updateMacro, UGKS_update and the cell's moments
+// one wavefront per cell: updateMacro, UGKS_update and the cell's moments.
+// A workgroup takes a 2 x 2 tile of cells (tiles j-fastest), so the north
+// S flux and the east W flux of half its cells are the other waves' own.
 template <typename T>
 __global__ void __launch_bounds__(kT) k_update(const FBlock *__restrict__ blocks, const long long *__restrict__ off,
-                                               int nb, long long ncells, const DtState *__restrict__ ds) {
+                                               int nb, long long ntiles, const DtState *__restrict__ ds) {
     __shared__ double tabs[kWavesT][4][64];
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const long long item = xcd_remap(blockIdx.x, gridDim.x) * kWavesT + wv;
-    if (item >= ncells) return;
+    const long long tile = xcd_remap(blockIdx.x, gridDim.x);
+    if (tile >= ntiles) return;
     const int lane = threadIdx.x & 63;
-    const int b = find_block(off, nb, item);
+    const int b = find_block(off, nb, tile);
     const FBlock &B = blocks[b];
     const Consts &C = B.c;
-    const long long cell = item - off[b];
-    const int mi = (int)(cell / C.nj), mj = (int)(cell % C.nj), gi = mi + C.G, gj = mj + C.G;
+    const long long t = tile - off[b];
+    const int nrp = (C.nj + 1) >> 1;
+    const int mi = 2 * (int)(t / nrp) + (wv >> 1), mj = 2 * (int)(t % nrp) + (wv & 1);
+    if (mi >= C.ni || mj >= C.nj) return;
+    const long long cell = (long long)mi * C.nj + mj;
+    const int gi = mi + C.G, gj = mj + C.G;
     const size_t c0 = (size_t)gi * C.NJ + gj;
     const double dt = ds[B.cs].dt;
     // updateMacro (clUGKS.c:1158-1187)
@@ -1228,7 +1234,8 @@ struct ugks_step_plan {
     int device;
     FBlock *d_blocks;
     long long *d_face_off, *d_cell_off, *d_all_off;   // prefix offsets: interior faces, cells, all cells
-    long long nall;
+    long long *d_tile_off;                              // prefix offsets of the update's 2 x 2 cell tiles
+    long long nall, ntiles;
     d4 *d_cmom;                                         // cell moments of every block, back to back
     WallItem *d_walls;
     GhostItem *d_ghosts;
@@ -1269,8 +1276,8 @@ template <typename T, int NPT> int launch_step(ugks_step_plan *p, hipStream_t s,
         hipLaunchKernelGGL((k_wall<T, NPT>), dim3((unsigned)p->nwalls), dim3(kT), 0, s, p->d_blocks, p->d_walls,
                            p->nwall_items, p->d_state);
     if (ev) (void)hipEventRecord(ev[4], s);
-    hipLaunchKernelGGL((k_update<T>), dim3((unsigned)((p->ncells + kWavesT - 1) / kWavesT)), dim3(kT), 0, s,
-                       p->d_blocks, p->d_cell_off, p->nb, p->ncells, p->d_state);
+    hipLaunchKernelGGL((k_update<T>), dim3((unsigned)p->ntiles), dim3(kT), 0, s, p->d_blocks, p->d_tile_off, p->nb,
+                       p->ntiles, p->d_state);
     if (ev) (void)hipEventRecord(ev[5], s);
     hipLaunchKernelGGL(k_dt, dim3((unsigned)((p->ncells + kT - 1) / kT)), dim3(kT), 0, s, p->d_blocks, p->d_cell_off,
                        p->nb, p->d_state, p->d_maxbits);
@@ -1338,7 +1345,8 @@ int ugks_step_plan_create_batch(ugks_block *const *blocks, const ugks_block_buff
     p->ncases = ncases;
     p->precision = precision;
     int maxnv = 0;
-    std::vector<long long> face_off(nblocks + 1, 0), cell_off(nblocks + 1, 0), all_off(nblocks + 1, 0);
+    std::vector<long long> face_off(nblocks + 1, 0), cell_off(nblocks + 1, 0), all_off(nblocks + 1, 0),
+        tile_off(nblocks + 1, 0);
     std::vector<WallItem> walls;
     std::vector<GhostItem> ghosts;
     long long nw = 0, ng = 0;
@@ -1395,6 +1403,7 @@ int ugks_step_plan_create_batch(ugks_block *const *blocks, const ugks_block_buff
         face_off[b + 1] = face_off[b] + F.nS + (long long)F.nWi * C.nj;
         cell_off[b + 1] = cell_off[b] + (long long)C.ni * C.nj;
         all_off[b + 1] = all_off[b] + (long long)C.NI * C.NJ;
+        tile_off[b + 1] = tile_off[b] + (long long)((C.ni + 1) / 2) * ((C.nj + 1) / 2);
         for (int k = 0; k < 4; k++) {
             int len = (k == GNORTH || k == GSOUTH) ? C.ni : C.nj;
             if (wallf[k]) { walls.push_back(WallItem{b, k, nw}); nw += len; }
@@ -1449,6 +1458,7 @@ int ugks_step_plan_create_batch(ugks_block *const *blocks, const ugks_block_buff
     p->nfaces = face_off[nblocks];
     p->ncells = cell_off[nblocks];
     p->nall = all_off[nblocks];
+    p->ntiles = tile_off[nblocks];
     p->nwalls = nw;
     p->nghosts = ng;
     p->nwall_items = (int)walls.size();
@@ -1460,6 +1470,7 @@ int ugks_step_plan_create_batch(ugks_block *const *blocks, const ugks_block_buff
     (void)alloc((void **)&p->d_face_off, sizeof(long long) * (nblocks + 1));
     (void)alloc((void **)&p->d_cell_off, sizeof(long long) * (nblocks + 1));
     (void)alloc((void **)&p->d_all_off, sizeof(long long) * (nblocks + 1));
+    (void)alloc((void **)&p->d_tile_off, sizeof(long long) * (nblocks + 1));
     (void)alloc((void **)&p->d_cmom, sizeof(d4) * (size_t)p->nall);
     (void)alloc((void **)&p->d_walls, sizeof(WallItem) * (walls.size() ? walls.size() : 1));
     (void)alloc((void **)&p->d_ghosts, sizeof(GhostItem) * ghosts.size());
@@ -1481,6 +1492,7 @@ int ugks_step_plan_create_batch(ugks_block *const *blocks, const ugks_block_buff
     if (e == hipSuccess) e = hipMemcpy(p->d_face_off, face_off.data(), sizeof(long long) * (nblocks + 1), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(p->d_cell_off, cell_off.data(), sizeof(long long) * (nblocks + 1), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(p->d_all_off, all_off.data(), sizeof(long long) * (nblocks + 1), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(p->d_tile_off, tile_off.data(), sizeof(long long) * (nblocks + 1), hipMemcpyHostToDevice);
     if (e == hipSuccess && walls.size())
         e = hipMemcpy(p->d_walls, walls.data(), sizeof(WallItem) * walls.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(p->d_ghosts, ghosts.data(), sizeof(GhostItem) * ghosts.size(), hipMemcpyHostToDevice);
@@ -1503,7 +1515,7 @@ int ugks_step_plan_destroy(ugks_step_plan *p) {
     if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
     if (p->graph) (void)hipGraphDestroy(p->graph);
     if (p->cap_stream) (void)hipStreamDestroy(p->cap_stream);
-    void *ptrs[] = {p->d_blocks, p->d_face_off, p->d_cell_off, p->d_all_off, p->d_cmom, p->d_walls, p->d_ghosts,
+    void *ptrs[] = {p->d_blocks, p->d_face_off, p->d_cell_off, p->d_all_off, p->d_tile_off, p->d_cmom, p->d_walls, p->d_ghosts,
                     p->d_state, p->d_case_first, p->d_maxbits, p->d_max_dt, p->d_fsum, p->d_fstate};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
