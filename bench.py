@@ -148,7 +148,9 @@ def run_precision(gs, precision, a, upd_per_step, workload):
     sims = [UGKSim(copy.deepcopy(g)) for g in gs]
     mass0 = [sum(s.total_mass()) for s in sims]
     plan = StepPlan(sims, precision, graph=not a.no_graph)
-    print(f"[bench] {precision}: {len(plan.blocks)} blocks in {len(sims)} case(s)", file=sys.stderr, flush=True)
+    tuned = plan.autotune()
+    print(f"[bench] {precision}: {len(plan.blocks)} blocks in {len(sims)} case(s), tuned {tuned}",
+          file=sys.stderr, flush=True)
     wall_s, ev_s = timed_run(plan, a.steps, a.warmup)
     value = upd_per_step * a.steps / wall_s
     print(f"[bench] {precision}: {a.steps} steps in {wall_s:.4f} s ({ev_s:.4f} s by events)",
@@ -172,7 +174,7 @@ def run_precision(gs, precision, a, upd_per_step, workload):
               "mass": sum(s.total_mass())} for s in sims]
     rep = {"value": value, "ms_per_step": 1e3 * wall_s / a.steps, "roofline": roof,
            "kernel_ms_per_launch": per_launch, "flag": plan.flag(),
-           "steps_done": [d.step for d in states],
+           "steps_done": [d.step for d in states], "autotune": tuned,
            "mass_drift": max(abs(f["mass"] - m0) / m0 for f, m0 in zip(final, mass0)),
            "graph": not a.no_graph}
     plan.close()
@@ -272,6 +274,7 @@ def main():
                    "Nv": sorted(nv_set), "path": f"fused {head}" + (" graph" if r["graph"] else "")},
         "roofline": r["roofline"],
         "kernel_ms_per_launch": r["kernel_ms_per_launch"],
+        "autotune": r["autotune"],
         "mass_drift": r["mass_drift"],
         "flag": r["flag"],
     }

@@ -244,6 +244,13 @@ int ugks_step_run_graph(ugks_step_plan *plan, void *stream, int32_t nsteps);
 /* the same steps with HIP events around each launch; kernel_ms[UGKS_STEP_NKERNELS]
  * receives the summed time of each launch of the step, in the order above */
 int ugks_step_run_timed(ugks_step_plan *plan, void *stream, int32_t nsteps, float *kernel_ms);
+/* Time the face passes with each workgroup size (1, 2 or 4 faces per
+ * workgroup, reps launches each) and keep the fastest per pass; choice[2]
+ * (may be NULL) receives the picks.  It leaves the simulation state as it was:
+ * the face passes only rewrite per-step scratch.  No reference counterpart:
+ * the reference takes CL_local_size from its own timing loop
+ * (ugks_sim.py:352-405). */
+int ugks_step_autotune(ugks_step_plan *plan, void *stream, int32_t reps, int32_t *choice);
 /* read back dt, time, step of case 0 (and the per-block maximum time steps if max_dt != NULL) */
 int ugks_step_state(ugks_step_plan *plan, void *stream, ugks_dt_desc *out, double *max_dt);
 /* the same for every case: out[ncases], max_dt[nblocks] (may be NULL) */

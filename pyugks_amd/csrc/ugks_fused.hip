@@ -47,20 +47,8 @@ namespace {
 constexpr int kT = 256;           // threads per workgroup
 constexpr int kWavesT = kT / 64;
 
-// wavefront-per-face kernels: workgroup size and an optional occupancy hint
-#ifndef UGKS_FACE_WG
-#define UGKS_FACE_WG 256
-#endif
-#ifndef UGKS_FACE_WPE
-#define UGKS_FACE_WPE 0
-#endif
-constexpr int kFW = UGKS_FACE_WG;
-constexpr int kFWaves = kFW / 64;
-#if UGKS_FACE_WPE
-#define FACE_KERNEL __global__ void __launch_bounds__(kFW) __attribute__((amdgpu_waves_per_eu(UGKS_FACE_WPE)))
-#else
-#define FACE_KERNEL __global__ void __launch_bounds__(kFW)
-#endif
+// wavefront-per-face kernels come in workgroups of FW = 1, 2 or 4 wavefronts
+// (faces); the plan's autotuner picks one per kernel (ugks_step_autotune)
 
 template <typename T> struct __align__(2 * sizeof(T)) P2 { T x, y; };
 
@@ -400,10 +388,10 @@ __device__ __forceinline__ FaceLoc face_loc(const FBlock &B, long long loc) {
     return f;
 }
 
-template <typename T>
-FACE_KERNEL k_faceA(const FBlock *__restrict__ blocks, const long long *__restrict__ off,
-                                              int nb, long long nfaces, double *__restrict__ fsum) {
-    const long long fid = xcd_remap(blockIdx.x, gridDim.x) * kFWaves + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+template <typename T, int FW>
+__global__ void __launch_bounds__(64 * FW) k_faceA(const FBlock *__restrict__ blocks, const long long *__restrict__ off,
+                                                   int nb, long long nfaces, double *__restrict__ fsum) {
+    const long long fid = xcd_remap(blockIdx.x, gridDim.x) * FW + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (fid >= nfaces) return;
     const int lane = threadIdx.x & 63;
     const int b = find_block(off, nb, fid);
@@ -499,12 +487,12 @@ __global__ void __launch_bounds__(kT) k_faceS(const FBlock *__restrict__ blocks,
     o[28] = B.side_length[own * 2 + L.face]; o[29] = n.x; o[30] = n.y; o[31] = 0.0;
 }
 
-template <typename T>
-FACE_KERNEL k_faceC(const FBlock *__restrict__ blocks, const long long *__restrict__ off,
-                                              int nb, long long nfaces, const double *__restrict__ fstate) {
-    __shared__ double tab[kFWaves][128];
+template <typename T, int FW>
+__global__ void __launch_bounds__(64 * FW) k_faceC(const FBlock *__restrict__ blocks, const long long *__restrict__ off,
+                                                   int nb, long long nfaces, const double *__restrict__ fstate) {
+    __shared__ double tab[FW][128];
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const long long fid = xcd_remap(blockIdx.x, gridDim.x) * kFWaves + wv;
+    const long long fid = xcd_remap(blockIdx.x, gridDim.x) * FW + wv;
     if (fid >= nfaces) return;
     const int lane = threadIdx.x & 63;
     const int b = find_block(off, nb, fid);
@@ -1236,6 +1224,7 @@ struct ugks_step_plan {
     long long *d_face_off, *d_cell_off, *d_all_off;   // prefix offsets: interior faces, cells, all cells
     long long *d_tile_off;                              // prefix offsets of the update's 2 x 2 cell tiles
     long long nall, ntiles;
+    int fa_waves, fc_waves;               // workgroup size of the face passes, in wavefronts (1, 2 or 4)
     d4 *d_cmom;                                         // cell moments of every block, back to back
     WallItem *d_walls;
     GhostItem *d_ghosts;
@@ -1254,23 +1243,35 @@ struct ugks_step_plan {
 };
 
 namespace {
+template <typename T, int FW> void launch_faceA(ugks_step_plan *p, hipStream_t s) {
+    hipLaunchKernelGGL((k_faceA<T, FW>), dim3((unsigned)((p->nfaces + FW - 1) / FW)), dim3(64 * FW), 0, s,
+                       p->d_blocks, p->d_face_off, p->nb, p->nfaces, p->d_fsum);
+}
+template <typename T, int FW> void launch_faceC(ugks_step_plan *p, hipStream_t s) {
+    hipLaunchKernelGGL((k_faceC<T, FW>), dim3((unsigned)((p->nfaces + FW - 1) / FW)), dim3(64 * FW), 0, s,
+                       p->d_blocks, p->d_face_off, p->nb, p->nfaces, (const double *)p->d_fstate);
+}
+template <typename T> void faceA(ugks_step_plan *p, hipStream_t s, int fw) {
+    if (fw == 1) launch_faceA<T, 1>(p, s); else if (fw == 2) launch_faceA<T, 2>(p, s); else launch_faceA<T, 4>(p, s);
+}
+template <typename T> void faceC(ugks_step_plan *p, hipStream_t s, int fw) {
+    if (fw == 1) launch_faceC<T, 1>(p, s); else if (fw == 2) launch_faceC<T, 2>(p, s); else launch_faceC<T, 4>(p, s);
+}
+
 // ev != nullptr: record ev[0..kLaunches] around the launches (per-kernel timing)
 constexpr int kLaunches = UGKS_STEP_NKERNELS;
 template <typename T, int NPT> int launch_step(ugks_step_plan *p, hipStream_t s, hipEvent_t *ev) {
-    const unsigned gw = (unsigned)((p->nfaces + kFWaves - 1) / kFWaves);   // one wavefront per face
     const unsigned gt = (unsigned)((p->nfaces + kT - 1) / kT);             // one thread per face
     if (ev) (void)hipEventRecord(ev[0], s);
     if (p->nfaces > 0)
-        hipLaunchKernelGGL((k_faceA<T>), dim3(gw), dim3(kFW), 0, s, p->d_blocks, p->d_face_off, p->nb, p->nfaces,
-                           p->d_fsum);
+        faceA<T>(p, s, p->fa_waves);
     if (ev) (void)hipEventRecord(ev[1], s);
     if (p->nfaces > 0)
         hipLaunchKernelGGL(k_faceS, dim3(gt), dim3(kT), 0, s, p->d_blocks, p->d_face_off, p->nb, p->nfaces,
                            (const double *)p->d_fsum, p->d_fstate, (const DtState *)p->d_state);
     if (ev) (void)hipEventRecord(ev[2], s);
     if (p->nfaces > 0)
-        hipLaunchKernelGGL((k_faceC<T>), dim3(gw), dim3(kFW), 0, s, p->d_blocks, p->d_face_off, p->nb, p->nfaces,
-                           (const double *)p->d_fstate);
+        faceC<T>(p, s, p->fc_waves);
     if (ev) (void)hipEventRecord(ev[3], s);
     if (p->nwalls > 0)
         hipLaunchKernelGGL((k_wall<T, NPT>), dim3((unsigned)p->nwalls), dim3(kT), 0, s, p->d_blocks, p->d_walls,
@@ -1317,6 +1318,52 @@ template <typename T> int dispatch_npt(ugks_step_plan *p, hipStream_t s, hipEven
 }
 }  // namespace
 
+namespace {
+template <typename T> int autotune(ugks_step_plan *p, hipStream_t s, int reps, int32_t *choice) {
+    // The face passes only read f and the step state and rewrite per-step scratch
+    // (face sums, face states, fluxes, Q from the same f), so timing them
+    // repeatedly leaves the simulation state as it was.
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+        return ugks_rt::fail(-1, "autotune: event create");
+    const int cand[3] = {1, 2, 4};
+    float best[2] = {1e30f, 1e30f};
+    int pick[2] = {4, 4};
+    int rc = 0;
+    for (int which = 0; which < 2 && !rc; which++)
+        for (int c = 0; c < 3 && !rc; c++) {
+            const int fw = cand[c];
+            // kC reads kA's results through kS: run the chain once, then time the kernel alone
+            faceA<T>(p, s, which == 0 ? fw : p->fa_waves);
+            hipLaunchKernelGGL(k_faceS, dim3((unsigned)((p->nfaces + kT - 1) / kT)), dim3(kT), 0, s, p->d_blocks,
+                               p->d_face_off, p->nb, p->nfaces, (const double *)p->d_fsum, p->d_fstate,
+                               (const DtState *)p->d_state);
+            (void)hipEventRecord(e0, s);
+            for (int r = 0; r < reps; r++) {
+                if (which == 0) faceA<T>(p, s, fw);
+                else faceC<T>(p, s, fw);
+            }
+            (void)hipEventRecord(e1, s);
+            if ((rc = ugks_rt::check_launch("autotune"))) break;
+            if (hipEventSynchronize(e1) != hipSuccess) { rc = ugks_rt::fail(-1, "autotune: sync"); break; }
+            float ms = 0.0f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            if (ms < best[which]) { best[which] = ms; pick[which] = fw; }
+            if (which == 0 && c == 2) p->fa_waves = pick[0];
+        }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    p->fa_waves = pick[0];
+    p->fc_waves = pick[1];
+    if (choice) { choice[0] = pick[0]; choice[1] = pick[1]; }
+    // a captured graph holds the old launch shapes
+    if (p->graph_exec) { (void)hipGraphExecDestroy(p->graph_exec); p->graph_exec = nullptr; }
+    if (p->graph) { (void)hipGraphDestroy(p->graph); p->graph = nullptr; }
+    return 0;
+}
+}  // namespace
+
 extern "C" {
 
 int ugks_step_plan_create_batch(ugks_block *const *blocks, const ugks_block_buffers *bufs, const ugks_bc_desc *bcs,
@@ -1343,6 +1390,7 @@ int ugks_step_plan_create_batch(ugks_block *const *blocks, const ugks_block_buff
     ugks_step_plan *p = new ugks_step_plan();
     p->nb = nblocks;
     p->ncases = ncases;
+    p->fa_waves = p->fc_waves = 4;
     p->precision = precision;
     int maxnv = 0;
     std::vector<long long> face_off(nblocks + 1, 0), cell_off(nblocks + 1, 0), all_off(nblocks + 1, 0),
@@ -1634,6 +1682,14 @@ int ugks_step_run_graph(ugks_step_plan *p, void *stream, int32_t nsteps) {
         if (e != hipSuccess) return ugks_rt::hip_fail(e, "step graph: launch");
     }
     return 0;
+}
+
+
+int ugks_step_autotune(ugks_step_plan *p, void *stream, int32_t reps, int32_t *choice) {
+    if (!p || reps < 1) return ugks_rt::fail(-1, "autotune: bad arguments");
+    if (p->nfaces == 0) return 0;
+    hipStream_t s = (hipStream_t)stream;
+    return p->precision == UGKS_FP64 ? autotune<double>(p, s, reps, choice) : autotune<float>(p, s, reps, choice);
 }
 
 int ugks_copy_kernel(void *stream, const void *src, void *dst, int64_t bytes) {

@@ -89,6 +89,12 @@ class StepPlan:
         else:
             L.check(lib.ugks_step_run(self.handle, self.stream, int(nsteps)), "ugks_step_run")
 
+    def autotune(self, reps: int = 3) -> dict:
+        """Pick the face passes' workgroup sizes by timing them (state unchanged)."""
+        choice = (C.c_int32 * 2)()
+        L.check(L.load().ugks_step_autotune(self.handle, self.stream, int(reps), choice), "ugks_step_autotune")
+        return {"k_faceA_waves_per_workgroup": int(choice[0]), "k_faceC_waves_per_workgroup": int(choice[1])}
+
     def run_timed(self, nsteps: int) -> dict:
         """Run nsteps with HIP events around each launch; per-kernel total ms."""
         ms = (C.c_float * len(self.KERNELS))()

@@ -67,6 +67,7 @@ FUSED = {
     "ugks_step_plan_destroy": [P],
     "ugks_step_run": [P, P, I],
     "ugks_step_run_graph": [P, P, I],
+    "ugks_step_autotune": [P, P, I, P],
     "ugks_step_run_timed": [P, P, I, P],
     "ugks_step_state": [P, P, P, P],
     "ugks_step_states": [P, P, P, P],

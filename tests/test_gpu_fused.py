@@ -141,3 +141,27 @@ def test_graph_replays_match_eager_steps(hip, precision):
             assert np.array_equal(be.macro(), bg.macro()), rep
             assert np.array_equal(eager.f64(be), graph.f64(bg)), rep
             assert np.array_equal(be.Q_D.download(), bg.Q_D.download()), rep
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_autotune_keeps_state_and_results(hip, precision):
+    """Autotuning the face passes leaves the state alone, and the tuned plan
+    steps bit for bit like the untuned one (the workgroup size only groups
+    independent faces)."""
+    from pyugks_amd.fused import StepPlan
+    from pyugks_amd.sim import UGKSim
+    g = cases.cavity(16, ("NC", 8), "WENO5", 0.01, nb=(2, 1))
+    s1, s2 = UGKSim(copy.deepcopy(g)), UGKSim(copy.deepcopy(g))
+    p1, p2 = StepPlan(s1, precision), StepPlan(s2, precision, graph=True)
+    p1.run(2)
+    p2.run(2)
+    f_before = [p2.f64(b) for b in s2.blocks]
+    choice = p2.autotune(reps=2)
+    assert set(choice.values()) <= {1, 2, 4}
+    assert all(np.array_equal(a, p2.f64(b)) for a, b in zip(f_before, s2.blocks))
+    p1.run(3)
+    p2.run(3)
+    assert p1.state().dt == p2.state().dt
+    for b1, b2 in zip(s1.blocks, s2.blocks):
+        assert np.array_equal(b1.macro(), b2.macro())
+        assert np.array_equal(p1.f64(b1), p2.f64(b2))
