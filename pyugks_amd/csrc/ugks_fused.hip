@@ -240,8 +240,13 @@ template <typename T> struct Stencil {
     const AS1 char *base;
     unsigned sb;
 };
-template <typename T> __device__ __forceinline__ Stencil<T> stencil(const Consts &C, const void *F, const FaceGeo &g) {
-    const int R = C.method == 1 ? 3 : 2;
+// M: the reconstruction method when the kernel is specialised for one
+// (0 vanLeer, 1 WENO5, 2 NND), -1 to read it from the block
+template <int M> __device__ __forceinline__ int method_of(const Consts &C) { return M >= 0 ? M : C.method; }
+
+template <typename T, int M = -1>
+__device__ __forceinline__ Stencil<T> stencil(const Consts &C, const void *F, const FaceGeo &g) {
+    const int R = method_of<M>(C) == 1 ? 3 : 2;
     Stencil<T> st;
     st.base = (const AS1 char *)F + ((long long)g.own - R * g.step) * (long long)C.NV * (long long)sizeof(P2<T>);
     st.sb = (unsigned)(g.step * C.NV * (long long)sizeof(P2<T>));
@@ -257,10 +262,10 @@ template <typename T> __device__ __forceinline__ P2<T> sld(const Stencil<T> &st,
 // WENO5, 3 otherwise), stencil index order; named members keep them in
 // registers across the load / compute split
 template <typename T> struct StenVals { P2<T> s0, s1, s2, s3, s4; };
-template <typename T>
+template <typename T, int M = -1>
 __device__ __forceinline__ void stencil_load(const Consts &C, const Stencil<T> &st, int v, int delta, StenVals<T> &x) {
     const unsigned vb = (unsigned)v * (unsigned)sizeof(P2<T>);
-    if (C.method == 1) {
+    if (method_of<M>(C) == 1) {
         x.s0 = sld<T>(st, delta ? 0 : 5, vb);
         x.s1 = sld<T>(st, delta ? 1 : 4, vb);
         x.s2 = sld<T>(st, delta ? 2 : 3, vb);
@@ -276,10 +281,10 @@ __device__ __forceinline__ void stencil_load(const Consts &C, const Stencil<T> &
 
 // reconstructed face value and slope (iFace) from the stencil values; fl /
 // fr, the cells below / above the face, on request
-template <typename T>
+template <typename T, int M = -1>
 __device__ __forceinline__ void reconstruct_from(const Consts &C, const FaceGeo &g, int delta, const StenVals<T> &x,
                                                  P2<T> &ff, P2<T> &sg, P2<T> *fl = nullptr, P2<T> *fr = nullptr) {
-    if (C.method == 1) {
+    if (method_of<M>(C) == 1) {
         const T dist = (T)(delta ? g.dist[1] : g.dist[0]), rdx = (T)(delta ? g.rdist[1] : g.rdist[0]);
         weno5_pair(x.s0, x.s1, x.s2, x.s3, x.s4, rdx, sg.x, sg.y);
         ff.x = x.s2.x + sg.x * dist;
@@ -288,7 +293,7 @@ __device__ __forceinline__ void reconstruct_from(const Consts &C, const FaceGeo 
         if (fr) *fr = P2<T>{delta ? x.s3.x : x.s2.x, delta ? x.s3.y : x.s2.y};
     } else {
         const T dist = (T)(delta ? g.dist[1] : g.dist[0]);
-        if (C.method == 0) {
+        if (method_of<M>(C) == 0) {
             const T r1 = (T)(delta ? g.rd1[1] : g.rd1[0]);
             sg.x = vanleer_T<T>((x.s1.x - x.s0.x) * r1, (x.s2.x - x.s1.x) * r1);
             sg.y = vanleer_T<T>((x.s1.y - x.s0.y) * r1, (x.s2.y - x.s1.y) * r1);
@@ -318,7 +323,7 @@ __device__ __forceinline__ void reconstruct(const Consts &C, const Stencil<T> &s
 #ifndef UGKS_PREFETCH
 #define UGKS_PREFETCH 0
 #endif
-template <typename T, typename Body>
+template <typename T, int M, typename Body>
 __device__ __forceinline__ void face_velocity_loop(const Consts &C, const Stencil<T> &st, d2 n, int lane, Body body) {
 #if UGKS_PREFETCH
     int v = lane;
@@ -326,7 +331,7 @@ __device__ __forceinline__ void face_velocity_loop(const Consts &C, const Stenci
     d2 uv = face_velocity(C, v, n);
     int delta = upwind(uv.x);
     StenVals<T> cur;
-    stencil_load<T>(C, st, v, delta, cur);
+    stencil_load<T, M>(C, st, v, delta, cur);
     while (true) {
         const int vn = v + 64;
         const bool more = vn < C.NV;
@@ -336,7 +341,7 @@ __device__ __forceinline__ void face_velocity_loop(const Consts &C, const Stenci
         if (more) {
             uvn = face_velocity(C, vn, n);
             dn = upwind(uvn.x);
-            stencil_load<T>(C, st, vn, dn, nxt);
+            stencil_load<T, M>(C, st, vn, dn, nxt);
         }
         body(v, uv, delta, cur);
         if (!more) break;
@@ -347,7 +352,7 @@ __device__ __forceinline__ void face_velocity_loop(const Consts &C, const Stenci
         const d2 uv = face_velocity(C, v, n);
         const int delta = upwind(uv.x);
         StenVals<T> cur;
-        stencil_load<T>(C, st, v, delta, cur);
+        stencil_load<T, M>(C, st, v, delta, cur);
         body(v, uv, delta, cur);
     }
 #endif
@@ -388,7 +393,7 @@ __device__ __forceinline__ FaceLoc face_loc(const FBlock &B, long long loc) {
     return f;
 }
 
-template <typename T, int FW>
+template <typename T, int FW, int M>
 __global__ void __launch_bounds__(64 * FW) k_faceA(const FBlock *__restrict__ blocks, const long long *__restrict__ off,
                                                    int nb, long long nfaces, double *__restrict__ fsum) {
     const long long fid = xcd_remap(blockIdx.x, gridDim.x) * FW + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -399,7 +404,7 @@ __global__ void __launch_bounds__(64 * FW) k_faceA(const FBlock *__restrict__ bl
     const Consts &C = B.c;
     const FaceLoc L = face_loc(B, fid - off[b]);
     const FaceGeo g = face_geo(B, L.gi, L.gj, L.face);
-    const Stencil<T> stn = stencil<T>(C, B.f, g);
+    const Stencil<T> stn = stencil<T, M>(C, B.f, g);
     const d2 n = B.normal[g.own * 2 + L.face];
     const bool wantQ = (L.face == SOUTH) && (L.mj < C.nj);
     const d4 cm = wantQ ? B.macro[(size_t)L.mi * C.nj + L.mj] : mk4(0, 0, 0, 1);
@@ -407,9 +412,9 @@ __global__ void __launch_bounds__(64 * FW) k_faceA(const FBlock *__restrict__ bl
     double a[kNS];
 #pragma unroll
     for (int q = 0; q < kNS; q++) a[q] = 0.0;
-    face_velocity_loop<T>(C, stn, n, lane, [&](int v, d2 uv, int delta, const StenVals<T> &sv) __attribute__((always_inline)) {
+    face_velocity_loop<T, M>(C, stn, n, lane, [&](int v, d2 uv, int delta, const StenVals<T> &sv) __attribute__((always_inline)) {
         P2<T> ff, sg, fl, fr;
-        reconstruct_from<T>(C, g, delta, sv, ff, sg, &fl, &fr);
+        reconstruct_from<T, M>(C, g, delta, sv, ff, sg, &fl, &fr);
         const double u = uv.x, w = uv.y, gx = (double)ff.x, hx = (double)ff.y;
         const double ug = u * gx, wg = w * gx, uug = u * ug, wwg = w * wg;
         a[0] += gx; a[1] += ug; a[2] += wg;
@@ -487,7 +492,7 @@ __global__ void __launch_bounds__(kT) k_faceS(const FBlock *__restrict__ blocks,
     o[28] = B.side_length[own * 2 + L.face]; o[29] = n.x; o[30] = n.y; o[31] = 0.0;
 }
 
-template <typename T, int FW>
+template <typename T, int FW, int M>
 __global__ void __launch_bounds__(64 * FW) k_faceC(const FBlock *__restrict__ blocks, const long long *__restrict__ off,
                                                    int nb, long long nfaces, const double *__restrict__ fstate) {
     __shared__ double tab[FW][128];
@@ -500,7 +505,7 @@ __global__ void __launch_bounds__(64 * FW) k_faceC(const FBlock *__restrict__ bl
     const Consts &C = B.c;
     const FaceLoc L = face_loc(B, fid - off[b]);
     const FaceGeo g = face_geo(B, L.gi, L.gj, L.face);
-    const Stencil<T> stn = stencil<T>(C, B.f, g);
+    const Stencil<T> stn = stencil<T, M>(C, B.f, g);
     const double *sv = fstate + fid * kFT;
     const d4 prim = mk4(sv[0], sv[1], sv[2], sv[3]);
     const d4 aL = mk4(sv[4], sv[5], sv[6], sv[7]), aR = mk4(sv[8], sv[9], sv[10], sv[11]);
@@ -544,9 +549,9 @@ __global__ void __launch_bounds__(64 * FW) k_faceC(const FBlock *__restrict__ bl
     // tensor indices of v = lane + 64 k, advanced without integer division
     int iu = 0, jv = 0, diu = 0, djv = 0;
     if (sep) { iu = lane / C.nvt; jv = lane - iu * C.nvt; diu = 64 / C.nvt; djv = 64 - diu * C.nvt; }
-    face_velocity_loop<T>(C, stn, n, lane, [&](int v, d2 uv, int delta, const StenVals<T> &stv) __attribute__((always_inline)) {
+    face_velocity_loop<T, M>(C, stn, n, lane, [&](int v, d2 uv, int delta, const StenVals<T> &stv) __attribute__((always_inline)) {
         P2<T> ff, sg;
-        reconstruct_from<T>(C, g, delta, stv, ff, sg);
+        reconstruct_from<T, M>(C, g, delta, stv, ff, sg);
         const double ux = uv.x, uy = uv.y, cx = ux - prim.s1, cy = uy - prim.s2, c2 = cx * cx + cy * cy;
         const double E = sep ? Ei[iu] * Ej[jv] : exp(-prim.s3 * c2);
         if (sep) {
@@ -1225,6 +1230,7 @@ struct ugks_step_plan {
     long long *d_tile_off;                              // prefix offsets of the update's 2 x 2 cell tiles
     long long nall, ntiles;
     int fa_waves, fc_waves;               // workgroup size of the face passes, in wavefronts (1, 2 or 4)
+    int method;                           // the blocks' common reconstruction method, -1 if they differ
     d4 *d_cmom;                                         // cell moments of every block, back to back
     WallItem *d_walls;
     GhostItem *d_ghosts;
@@ -1243,19 +1249,36 @@ struct ugks_step_plan {
 };
 
 namespace {
-template <typename T, int FW> void launch_faceA(ugks_step_plan *p, hipStream_t s) {
-    hipLaunchKernelGGL((k_faceA<T, FW>), dim3((unsigned)((p->nfaces + FW - 1) / FW)), dim3(64 * FW), 0, s,
+template <typename T, int FW, int M> void launch_faceA(ugks_step_plan *p, hipStream_t s) {
+    hipLaunchKernelGGL((k_faceA<T, FW, M>), dim3((unsigned)((p->nfaces + FW - 1) / FW)), dim3(64 * FW), 0, s,
                        p->d_blocks, p->d_face_off, p->nb, p->nfaces, p->d_fsum);
 }
-template <typename T, int FW> void launch_faceC(ugks_step_plan *p, hipStream_t s) {
-    hipLaunchKernelGGL((k_faceC<T, FW>), dim3((unsigned)((p->nfaces + FW - 1) / FW)), dim3(64 * FW), 0, s,
+template <typename T, int FW, int M> void launch_faceC(ugks_step_plan *p, hipStream_t s) {
+    hipLaunchKernelGGL((k_faceC<T, FW, M>), dim3((unsigned)((p->nfaces + FW - 1) / FW)), dim3(64 * FW), 0, s,
                        p->d_blocks, p->d_face_off, p->nb, p->nfaces, (const double *)p->d_fstate);
 }
+// the face passes specialised for the plan's one reconstruction method (or generic)
+template <typename T, int FW> void faceA_m(ugks_step_plan *p, hipStream_t s) {
+    switch (p->method) {
+    case 0: launch_faceA<T, FW, 0>(p, s); break;
+    case 1: launch_faceA<T, FW, 1>(p, s); break;
+    case 2: launch_faceA<T, FW, 2>(p, s); break;
+    default: launch_faceA<T, FW, -1>(p, s); break;
+    }
+}
+template <typename T, int FW> void faceC_m(ugks_step_plan *p, hipStream_t s) {
+    switch (p->method) {
+    case 0: launch_faceC<T, FW, 0>(p, s); break;
+    case 1: launch_faceC<T, FW, 1>(p, s); break;
+    case 2: launch_faceC<T, FW, 2>(p, s); break;
+    default: launch_faceC<T, FW, -1>(p, s); break;
+    }
+}
 template <typename T> void faceA(ugks_step_plan *p, hipStream_t s, int fw) {
-    if (fw == 1) launch_faceA<T, 1>(p, s); else if (fw == 2) launch_faceA<T, 2>(p, s); else launch_faceA<T, 4>(p, s);
+    if (fw == 1) faceA_m<T, 1>(p, s); else if (fw == 2) faceA_m<T, 2>(p, s); else faceA_m<T, 4>(p, s);
 }
 template <typename T> void faceC(ugks_step_plan *p, hipStream_t s, int fw) {
-    if (fw == 1) launch_faceC<T, 1>(p, s); else if (fw == 2) launch_faceC<T, 2>(p, s); else launch_faceC<T, 4>(p, s);
+    if (fw == 1) faceC_m<T, 1>(p, s); else if (fw == 2) faceC_m<T, 2>(p, s); else faceC_m<T, 4>(p, s);
 }
 
 // ev != nullptr: record ev[0..kLaunches] around the launches (per-kernel timing)
@@ -1391,6 +1414,9 @@ int ugks_step_plan_create_batch(ugks_block *const *blocks, const ugks_block_buff
     p->nb = nblocks;
     p->ncases = ncases;
     p->fa_waves = p->fc_waves = 4;
+    p->method = blocks[0] ? blocks[0]->c.method : -1;
+    for (int b = 1; b < nblocks; b++)
+        if (blocks[b] && blocks[b]->c.method != p->method) p->method = -1;
     p->precision = precision;
     int maxnv = 0;
     std::vector<long long> face_off(nblocks + 1, 0), cell_off(nblocks + 1, 0), all_off(nblocks + 1, 0),
