@@ -348,6 +348,10 @@ __device__ __forceinline__ void face_velocity_loop(const Consts &C, const Stenci
         v = vn; uv = uvn; delta = dn; cur = nxt;
     }
 #else
+#ifndef UGKS_FACE_UNROLL
+#define UGKS_FACE_UNROLL 1
+#endif
+#pragma unroll UGKS_FACE_UNROLL
     for (int v = lane; v < C.NV; v += 64) {
         const d2 uv = face_velocity(C, v, n);
         const int delta = upwind(uv.x);
@@ -492,8 +496,16 @@ __global__ void __launch_bounds__(kT) k_faceS(const FBlock *__restrict__ blocks,
     o[28] = B.side_length[own * 2 + L.face]; o[29] = n.x; o[30] = n.y; o[31] = 0.0;
 }
 
+#ifndef UGKS_FACEC_WPE
+#define UGKS_FACEC_WPE 0
+#endif
+#if UGKS_FACEC_WPE
+#define FACEC_ATTR __attribute__((amdgpu_waves_per_eu(UGKS_FACEC_WPE)))
+#else
+#define FACEC_ATTR
+#endif
 template <typename T, int FW, int M>
-__global__ void __launch_bounds__(64 * FW) k_faceC(const FBlock *__restrict__ blocks, const long long *__restrict__ off,
+__global__ void __launch_bounds__(64 * FW) FACEC_ATTR k_faceC(const FBlock *__restrict__ blocks, const long long *__restrict__ off,
                                                    int nb, long long nfaces, const double *__restrict__ fstate) {
     __shared__ double tab[FW][128];
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
