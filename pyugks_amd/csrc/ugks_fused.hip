@@ -253,6 +253,37 @@ __device__ __forceinline__ Stencil<T> stencil(const Consts &C, const void *F, co
     return st;
 }
 template <typename T> using V2 = T __attribute__((ext_vector_type(2)));
+
+// velocity table reads through global (not generic) pointers: they count only
+// against the vector-memory counter, so waiting for them does not also wait on LDS
+__device__ __forceinline__ d2 quad_at(const Consts &C, int v) {
+    const V2<double> x = ((const AS1 V2<double> *)C.quad)[v];
+    return d2{x.x, x.y};
+}
+__device__ __forceinline__ double weight_at(const Consts &C, int v) { return ((const AS1 double *)C.weight)[v]; }
+__device__ __forceinline__ d2 face_velocity_g(const Consts &C, int v, d2 n) {
+    const d2 e = quad_at(C, v);
+    return d2{e.x * n.x + e.y * n.y, e.x * (-n.y) + e.y * n.x};
+}
+// a wavefront-uniform double held in scalar registers (keeps per-face
+// constants out of VGPRs, so they are neither spilled nor recomputed in loops)
+__device__ __forceinline__ double uniform_d(double x) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)b), hi = __builtin_amdgcn_readfirstlane((unsigned)(b >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+// the reciprocals go through an empty asm so the compiler cannot turn the
+// per-lane select of two of them back into a division inside the velocity loop
+__device__ __forceinline__ void uniform_geo(FaceGeo &g) {
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        g.dist[k] = uniform_d(g.dist[k]);
+        g.rdist[k] = uniform_d(g.rdist[k]);
+        g.rd1[k] = uniform_d(g.rd1[k]);
+        __asm__ volatile("" : "+v"(g.rdist[k]));
+        __asm__ volatile("" : "+v"(g.rd1[k]));
+    }
+}
 template <typename T> __device__ __forceinline__ P2<T> sld(const Stencil<T> &st, unsigned rel, unsigned vb) {
     const V2<T> x = *(const AS1 V2<T> *)(st.base + (rel * st.sb + vb));
     return P2<T>{x.x, x.y};
@@ -353,7 +384,7 @@ __device__ __forceinline__ void face_velocity_loop(const Consts &C, const Stenci
 #endif
 #pragma unroll UGKS_FACE_UNROLL
     for (int v = lane; v < C.NV; v += 64) {
-        const d2 uv = face_velocity(C, v, n);
+        const d2 uv = face_velocity_g(C, v, n);
         const int delta = upwind(uv.x);
         StenVals<T> cur;
         stencil_load<T, M>(C, st, v, delta, cur);
@@ -407,7 +438,8 @@ __global__ void __launch_bounds__(64 * FW) k_faceA(const FBlock *__restrict__ bl
     const FBlock &B = blocks[b];
     const Consts &C = B.c;
     const FaceLoc L = face_loc(B, fid - off[b]);
-    const FaceGeo g = face_geo(B, L.gi, L.gj, L.face);
+    FaceGeo g = face_geo(B, L.gi, L.gj, L.face);
+    uniform_geo(g);
     const Stencil<T> stn = stencil<T, M>(C, B.f, g);
     const d2 n = B.normal[g.own * 2 + L.face];
     const bool wantQ = (L.face == SOUTH) && (L.mj < C.nj);
@@ -426,7 +458,7 @@ __global__ void __launch_bounds__(64 * FW) k_faceA(const FBlock *__restrict__ bl
         a[6] += u * uug; a[7] += w * wwg; a[8] += u * wwg; a[9] += w * uug;
         a[10] += hx; a[11] += u * hx; a[12] += w * hx;
         if (wantQ) {
-            const d2 e = C.quad[v];
+            const d2 e = quad_at(C, v);
             const double x = (double)fr.x, y = (double)fr.y;
             const double cx = e.x - cm.s1, cy = e.y - cm.s2, t = (cx * cx + cy * cy) * x + y * rK;
             a[13] += 0.5 * (cx * t);
@@ -516,7 +548,8 @@ __global__ void __launch_bounds__(64 * FW) FACEC_ATTR k_faceC(const FBlock *__re
     const FBlock &B = blocks[b];
     const Consts &C = B.c;
     const FaceLoc L = face_loc(B, fid - off[b]);
-    const FaceGeo g = face_geo(B, L.gi, L.gj, L.face);
+    FaceGeo g = face_geo(B, L.gi, L.gj, L.face);
+    uniform_geo(g);
     const Stencil<T> stn = stencil<T, M>(C, B.f, g);
     const double *sv = fstate + fid * kFT;
     const d4 prim = mk4(sv[0], sv[1], sv[2], sv[3]);
@@ -570,7 +603,7 @@ __global__ void __launch_bounds__(64 * FW) FACEC_ATTR k_faceC(const FBlock *__re
             iu += diu; jv += djv;
             if (jv >= C.nvt) { jv -= C.nvt; iu++; }
         }
-        const double f0g = C.weight[v] * (rl * E);
+        const double f0g = weight_at(C, v) * (rl * E);
         const double sh = sfac * (cx * Qf.x + cy * Qf.y), part = 2 * prim.s3 * c2;
         const double uu = ux * ux + uy * uy, ux2 = ux * ux;
         const double PL = aL.s0 + aL.s1 * ux + aL.s2 * uy + hL * uu, PR = aR.s0 + aR.s1 * ux + aR.s2 * uy + hR * uu;
