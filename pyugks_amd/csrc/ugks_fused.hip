@@ -348,49 +348,30 @@ __device__ __forceinline__ void reconstruct(const Consts &C, const Stencil<T> &s
     reconstruct_from<T>(C, g, delta, x, ff, sg, fl, fr);
 }
 
-// Velocity loop of a wavefront-per-face pass, v = lane, lane + 64, ...:
-// the stencil of the next velocity is loaded before the current one is
-// used, so the loads overlap the arithmetic.
-#ifndef UGKS_PREFETCH
-#define UGKS_PREFETCH 0
-#endif
+// Velocity loop of a wavefront-per-face pass, v = lane, lane + 64, ...: the
+// body gets the global velocity e, its weight w, the face-frame velocity uv,
+// the upwind choice and the stencil.  The velocity and weight of the next
+// iteration are loaded one iteration ahead, so the stencil addresses (which
+// need uv) do not wait on a fresh load.  (Loading the whole next stencil ahead
+// was measured slower: the extra registers cost occupancy.)
 template <typename T, int M, typename Body>
 __device__ __forceinline__ void face_velocity_loop(const Consts &C, const Stencil<T> &st, d2 n, int lane, Body body) {
-#if UGKS_PREFETCH
-    int v = lane;
-    if (v >= C.NV) return;
-    d2 uv = face_velocity(C, v, n);
-    int delta = upwind(uv.x);
-    StenVals<T> cur;
-    stencil_load<T, M>(C, st, v, delta, cur);
-    while (true) {
-        const int vn = v + 64;
-        const bool more = vn < C.NV;
-        d2 uvn = uv;
-        int dn = delta;
-        StenVals<T> nxt;
-        if (more) {
-            uvn = face_velocity(C, vn, n);
-            dn = upwind(uvn.x);
-            stencil_load<T, M>(C, st, vn, dn, nxt);
-        }
-        body(v, uv, delta, cur);
-        if (!more) break;
-        v = vn; uv = uvn; delta = dn; cur = nxt;
-    }
-#else
-#ifndef UGKS_FACE_UNROLL
-#define UGKS_FACE_UNROLL 1
-#endif
-#pragma unroll UGKS_FACE_UNROLL
+    if (lane >= C.NV) return;
+    d2 e = quad_at(C, lane);
+    double w = weight_at(C, lane);
     for (int v = lane; v < C.NV; v += 64) {
-        const d2 uv = face_velocity_g(C, v, n);
+        const d2 ec = e;
+        const double wc = w;
+        if (v + 64 < C.NV) {
+            e = quad_at(C, v + 64);
+            w = weight_at(C, v + 64);
+        }
+        const d2 uv = d2{ec.x * n.x + ec.y * n.y, ec.x * (-n.y) + ec.y * n.x};     // face_velocity
         const int delta = upwind(uv.x);
         StenVals<T> cur;
         stencil_load<T, M>(C, st, v, delta, cur);
-        body(v, uv, delta, cur);
+        body(v, ec, wc, uv, delta, cur);
     }
-#endif
 }
 
 // the cells below (rel R-1) and above (rel R, the owner) the face
@@ -448,7 +429,7 @@ __global__ void __launch_bounds__(64 * FW) k_faceA(const FBlock *__restrict__ bl
     double a[kNS];
 #pragma unroll
     for (int q = 0; q < kNS; q++) a[q] = 0.0;
-    face_velocity_loop<T, M>(C, stn, n, lane, [&](int v, d2 uv, int delta, const StenVals<T> &sv) __attribute__((always_inline)) {
+    face_velocity_loop<T, M>(C, stn, n, lane, [&](int v, d2 e, double, d2 uv, int delta, const StenVals<T> &sv) __attribute__((always_inline)) {
         P2<T> ff, sg, fl, fr;
         reconstruct_from<T, M>(C, g, delta, sv, ff, sg, &fl, &fr);
         const double u = uv.x, w = uv.y, gx = (double)ff.x, hx = (double)ff.y;
@@ -458,7 +439,6 @@ __global__ void __launch_bounds__(64 * FW) k_faceA(const FBlock *__restrict__ bl
         a[6] += u * uug; a[7] += w * wwg; a[8] += u * wwg; a[9] += w * uug;
         a[10] += hx; a[11] += u * hx; a[12] += w * hx;
         if (wantQ) {
-            const d2 e = quad_at(C, v);
             const double x = (double)fr.x, y = (double)fr.y;
             const double cx = e.x - cm.s1, cy = e.y - cm.s2, t = (cx * cx + cy * cy) * x + y * rK;
             a[13] += 0.5 * (cx * t);
@@ -594,7 +574,7 @@ __global__ void __launch_bounds__(64 * FW) FACEC_ATTR k_faceC(const FBlock *__re
     // tensor indices of v = lane + 64 k, advanced without integer division
     int iu = 0, jv = 0, diu = 0, djv = 0;
     if (sep) { iu = lane / C.nvt; jv = lane - iu * C.nvt; diu = 64 / C.nvt; djv = 64 - diu * C.nvt; }
-    face_velocity_loop<T, M>(C, stn, n, lane, [&](int v, d2 uv, int delta, const StenVals<T> &stv) __attribute__((always_inline)) {
+    face_velocity_loop<T, M>(C, stn, n, lane, [&](int v, d2, double wv, d2 uv, int delta, const StenVals<T> &stv) __attribute__((always_inline)) {
         P2<T> ff, sg;
         reconstruct_from<T, M>(C, g, delta, stv, ff, sg);
         const double ux = uv.x, uy = uv.y, cx = ux - prim.s1, cy = uy - prim.s2, c2 = cx * cx + cy * cy;
@@ -603,7 +583,7 @@ __global__ void __launch_bounds__(64 * FW) FACEC_ATTR k_faceC(const FBlock *__re
             iu += diu; jv += djv;
             if (jv >= C.nvt) { jv -= C.nvt; iu++; }
         }
-        const double f0g = weight_at(C, v) * (rl * E);
+        const double f0g = wv * (rl * E);
         const double sh = sfac * (cx * Qf.x + cy * Qf.y), part = 2 * prim.s3 * c2;
         const double uu = ux * ux + uy * uy, ux2 = ux * ux;
         const double PL = aL.s0 + aL.s1 * ux + aL.s2 * uy + hL * uu, PR = aR.s0 + aR.s1 * ux + aR.s2 * uy + hR * uu;
