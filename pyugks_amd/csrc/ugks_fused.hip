@@ -348,28 +348,92 @@ __device__ __forceinline__ void reconstruct(const Consts &C, const Stencil<T> &s
     reconstruct_from<T>(C, g, delta, x, ff, sg, fl, fr);
 }
 
+#define AS3 __attribute__((address_space(3)))
+// LDS bytes of one wavefront's stencil staging: all 2R stencil cells (both
+// upwind choices) of 64 velocities.  FP64 only: a lane's (g, h) pair of floats
+// is 8 bytes, a size the LDS loads lack.  The flux pass stages its stencils
+// this way; the moment pass, which has no store in its loop and runs at a
+// higher occupancy, measured faster with plain loads.
+template <typename T> constexpr int kStenRow = 64 * (int)sizeof(P2<T>);
+template <typename T> constexpr bool kStenViaLds = sizeof(T) == 8;
+template <typename T> constexpr int kStenLds = kStenViaLds<T> ? 6 * kStenRow<T> : 16;
+
+// Loads of the 2R stencil cells of the velocities at byte offset vb, issued
+// straight into the wavefront's LDS buffer lb (no registers held while they
+// are in flight): cell r of lane l lands at lb + r * kStenRow + l * sizeof(P2).
+template <typename T, int M>
+__device__ __forceinline__ void stencil_dma(const Consts &C, const Stencil<T> &st, unsigned vb, AS3 char *lb) {
+    constexpr int sz = 16;   // one (g, h) pair of doubles per lane
+    const int ncell = method_of<M>(C) == 1 ? 6 : 4;
+#pragma unroll
+    for (int r = 0; r < 6; r++) {
+        if (r < ncell) {
+            const AS1 char *src = st.base + r * st.sb;
+            __builtin_amdgcn_global_load_lds((const AS1 void *)(src + vb), (AS3 void *)(lb + r * kStenRow<T>), sz, 0, 0);
+        }
+    }
+}
+template <typename T> __device__ __forceinline__ P2<T> lld(const AS3 char *p) {
+    const V2<T> x = *(const AS3 V2<T> *)p;
+    return P2<T>{x.x, x.y};
+}
+// the stencil of this lane's velocity from the staging buffer, in stencil
+// index order for upwind choice delta (cells 0..4 for delta = 1, 5..1 for 0)
+template <typename T, int M>
+__device__ __forceinline__ void stencil_read(const Consts &C, const AS3 char *lb, int lane, int delta, StenVals<T> &x) {
+    constexpr int row = kStenRow<T>;
+    const int last = method_of<M>(C) == 1 ? 5 : 3;
+    const AS3 char *p = lb + lane * (int)sizeof(P2<T>) + (delta ? 0 : last * row);
+    const int s = delta ? row : -row;
+    x.s0 = lld<T>(p);
+    x.s1 = lld<T>(p + s);
+    x.s2 = lld<T>(p + 2 * s);
+    if (method_of<M>(C) == 1) {
+        x.s3 = lld<T>(p + 3 * s);
+        x.s4 = lld<T>(p + 4 * s);
+    } else {
+        x.s3 = x.s4 = x.s2;
+    }
+}
+
 // Velocity loop of a wavefront-per-face pass, v = lane, lane + 64, ...: the
 // body gets the global velocity e, its weight w, the face-frame velocity uv,
-// the upwind choice and the stencil.  The velocity and weight of the next
-// iteration are loaded one iteration ahead, so the stencil addresses (which
-// need uv) do not wait on a fresh load.  (Loading the whole next stencil ahead
-// was measured slower: the extra registers cost occupancy.)
-template <typename T, int M, typename Body>
-__device__ __forceinline__ void face_velocity_loop(const Consts &C, const Stencil<T> &st, d2 n, int lane, Body body) {
+// the upwind choice and the stencil.  The stencil of the next velocity is
+// loaded into the wavefront's LDS buffer lb while the body of this one runs
+// (both upwind choices, so the loads need not wait for the next velocity);
+// the velocity and weight are loaded one iteration ahead into registers.
+template <typename T, int M, bool LDS, typename Body>
+__device__ __forceinline__ void face_velocity_loop(const Consts &C, const Stencil<T> &st, d2 n, int lane, AS3 char *lb,
+                                                   Body body) {
+    static_assert(!LDS || kStenViaLds<T>, "LDS stencil staging needs FP64 pairs");
     if (lane >= C.NV) return;
     d2 e = quad_at(C, lane);
     double w = weight_at(C, lane);
+    unsigned vb = (unsigned)lane * (unsigned)sizeof(P2<T>);
+    if constexpr (LDS) stencil_dma<T, M>(C, st, vb, lb);
     for (int v = lane; v < C.NV; v += 64) {
         const d2 ec = e;
         const double wc = w;
-        if (v + 64 < C.NV) {
-            e = quad_at(C, v + 64);
-            w = weight_at(C, v + 64);
-        }
         const d2 uv = d2{ec.x * n.x + ec.y * n.y, ec.x * (-n.y) + ec.y * n.x};     // face_velocity
         const int delta = upwind(uv.x);
         StenVals<T> cur;
-        stencil_load<T, M>(C, st, v, delta, cur);
+        if constexpr (LDS) {
+            stencil_read<T, M>(C, lb, lane, delta, cur);
+            // the reads have returned before the next loads overwrite the buffer
+            __builtin_amdgcn_s_waitcnt(0xC07F);   // lgkmcnt(0)
+            if (v + 64 < C.NV) {
+                e = quad_at(C, v + 64);
+                w = weight_at(C, v + 64);
+                vb += 64u * (unsigned)sizeof(P2<T>);
+                stencil_dma<T, M>(C, st, vb, lb);
+            }
+        } else {
+            if (v + 64 < C.NV) {
+                e = quad_at(C, v + 64);
+                w = weight_at(C, v + 64);
+            }
+            stencil_load<T, M>(C, st, v, delta, cur);
+        }
         body(v, ec, wc, uv, delta, cur);
     }
 }
@@ -429,7 +493,7 @@ __global__ void __launch_bounds__(64 * FW) k_faceA(const FBlock *__restrict__ bl
     double a[kNS];
 #pragma unroll
     for (int q = 0; q < kNS; q++) a[q] = 0.0;
-    face_velocity_loop<T, M>(C, stn, n, lane, [&](int v, d2 e, double, d2 uv, int delta, const StenVals<T> &sv) __attribute__((always_inline)) {
+    face_velocity_loop<T, M, false>(C, stn, n, lane, nullptr, [&](int v, d2 e, double, d2 uv, int delta, const StenVals<T> &sv) __attribute__((always_inline)) {
         P2<T> ff, sg, fl, fr;
         reconstruct_from<T, M>(C, g, delta, sv, ff, sg, &fl, &fr);
         const double u = uv.x, w = uv.y, gx = (double)ff.x, hx = (double)ff.y;
@@ -508,18 +572,16 @@ __global__ void __launch_bounds__(kT) k_faceS(const FBlock *__restrict__ blocks,
     o[28] = B.side_length[own * 2 + L.face]; o[29] = n.x; o[30] = n.y; o[31] = 0.0;
 }
 
-#ifndef UGKS_FACEC_WPE
-#define UGKS_FACEC_WPE 0
-#endif
-#if UGKS_FACEC_WPE
-#define FACEC_ATTR __attribute__((amdgpu_waves_per_eu(UGKS_FACEC_WPE)))
-#else
-#define FACEC_ATTR
-#endif
+// minimum wavefronts per SIMD asked of the flux pass: the FP32 variants with a
+// fixed reconstruction fit five without spilling; the others are left to the compiler
+template <typename T, int M> constexpr int kFaceCWaves = (sizeof(T) == 4 && M >= 0) ? 5 : 1;
+// the FP64 flux stores one iteration late (the FP32 variant would spill at five waves)
+template <typename T> constexpr bool kDeferStore = sizeof(T) == 8;
 template <typename T, int FW, int M>
-__global__ void __launch_bounds__(64 * FW) FACEC_ATTR k_faceC(const FBlock *__restrict__ blocks, const long long *__restrict__ off,
+__global__ void __launch_bounds__(64 * FW) __attribute__((amdgpu_waves_per_eu(kFaceCWaves<T, M>))) k_faceC(const FBlock *__restrict__ blocks, const long long *__restrict__ off,
                                                    int nb, long long nfaces, const double *__restrict__ fstate) {
     __shared__ double tab[FW][128];
+    __shared__ __attribute__((aligned(16))) char sbuf[FW][kStenLds<T>];
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const long long fid = xcd_remap(blockIdx.x, gridDim.x) * FW + wv;
     if (fid >= nfaces) return;
@@ -574,7 +636,15 @@ __global__ void __launch_bounds__(64 * FW) FACEC_ATTR k_faceC(const FBlock *__re
     // tensor indices of v = lane + 64 k, advanced without integer division
     int iu = 0, jv = 0, diu = 0, djv = 0;
     if (sep) { iu = lane / C.nvt; jv = lane - iu * C.nvt; diu = 64 / C.nvt; djv = 64 - diu * C.nvt; }
-    face_velocity_loop<T, M>(C, stn, n, lane, [&](int v, d2, double wv, d2 uv, int delta, const StenVals<T> &stv) __attribute__((always_inline)) {
+    // each velocity's flux is stored one iteration late, at the start of the
+    // next, so the wait for the loads issued there does not also wait for it
+    V2<T> pend = V2<T>{T(0), T(0)};
+    int last = -1;
+    face_velocity_loop<T, M, kStenViaLds<T>>(C, stn, n, lane, (AS3 char *)sbuf[wv], [&](int v, d2, double wv, d2 uv, int delta, const StenVals<T> &stv) __attribute__((always_inline)) {
+        if constexpr (kDeferStore<T>) {
+            if (v >= 64) fout[v - 64] = pend;
+            last = v;
+        }
         P2<T> ff, sg;
         reconstruct_from<T, M>(C, g, delta, stv, ff, sg);
         const double ux = uv.x, uy = uv.y, cx = ux - prim.s1, cy = uy - prim.s2, c2 = cx * cx + cy * cy;
@@ -595,7 +665,8 @@ __global__ void __launch_bounds__(64 * FW) FACEC_ATTR k_faceC(const FBlock *__re
         const double rg = f0g * (m0 * ux * (1.0 + shg) + m1 * ux2 * (Pa + ca) + m2 * ux * (Pt + ct)) + ux * tg;
         const double rh = f0g * (m0 * ux * hk * (1.0 + shh) + m1 * ux2 * (hk * Pa + cma) + m2 * ux * (hk * Pt + cmt)) +
                           ux * th;
-        fout[v] = V2<T>{(T)(Lf * rg), (T)(Lf * rh)};
+        if constexpr (kDeferStore<T>) pend = V2<T>{(T)(Lf * rg), (T)(Lf * rh)};
+        else fout[v] = V2<T>{(T)(Lf * rg), (T)(Lf * rh)};
         // macroFlux (clUGKS.c:607-661): the Shakhov and the non-equilibrium parts
         const double X = ux * (m0 * shg * f0g + tg);
         mf[0] += X;
@@ -603,6 +674,7 @@ __global__ void __launch_bounds__(64 * FW) FACEC_ATTR k_faceC(const FBlock *__re
         mf[2] += uy * X;
         mf[3] += 0.5 * (uu * X + ux * (m0 * shh * (hk * f0g) + th));
     });
+    if (last >= 0) fout[last] = pend;
 #pragma unroll
     for (int q = 0; q < 4; q++) mf[q] = wave_sum(mf[q]);
     if (lane == 0) {
