@@ -47,6 +47,7 @@ typedef struct ugks_block_desc {
     const double *quad;             /* host [NV][2]  (QUAD,   ugks_CL.py:318-333) */
     const double *weight;           /* host [NV]     (WEIGHT, ugks_CL.py:335-346) */
     const int64_t *mirror_NS, *mirror_EW, *mirror_D;  /* host [NV] (ugks_CL.py:348-395) */
+    double alpha_n[4], alpha_t[4];  /* Cercignani-Lampis accommodation, -1 off adsorbing walls (ugks_CL.py:134-148) */
 } ugks_block_desc;
 
 typedef struct ugks_block ugks_block;
@@ -146,6 +147,11 @@ int ugks_adsorbingWallS_P2(ugks_block *blk, void *stream, const void *normal, in
 int ugks_adsorbingWallD_P2(ugks_block *blk, void *stream, const void *normal, int32_t face,
                            const void *wall_prop, const void *wall_cover, const void *wall_dist,
                            void *flux_f, const void *macro, double dt);
+/* clAdsorb.c:488-495, ugks_block.py:954-957: Cercignani-Lampis re-emission,
+ * accommodation alpha_n / alpha_t from ugks_block_desc */
+int ugks_adsorbingWallCL_P2(ugks_block *blk, void *stream, const void *normal, int32_t face,
+                            const void *wall_prop, const void *wall_cover, const void *wall_dist,
+                            void *flux_f, const void *macro, double dt);
 /* clUGKS.c:1158-1163, ugks_block.py:1279-1281 */
 int ugks_updateMacro(ugks_block *blk, void *stream, const void *flux_macro_S,
                      const void *flux_macro_W, const void *area, void *macro, void *residual);

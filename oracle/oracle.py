@@ -45,6 +45,7 @@ class Params(C.Structure):
         ("beta_n", D * 4), ("beta_t", D * 4), ("gamma_f", D * 4), ("alpha_p", D * 4),
         ("dynamic_vartheta", I * 4), ("constant_adsorb", I * 4), ("adsorb", D * 4),
         ("quad", P), ("weight", P), ("mirror_NS", P), ("mirror_EW", P), ("mirror_D", P),
+        ("alpha_n", D * 4), ("alpha_t", D * 4),
     ]
 
 
@@ -116,6 +117,10 @@ def helpers():
         "or_microSlope": (V4, [PP, V4, V4]),
         "or_getPrimary": (V4, [PP, V4]),
         "or_getConserved": (V4, [PP, V4]),
+        "or_bessel_i0": (D, [D]),
+        "or_cl_kernel": (D, [V2, V2, D, D, D]),
+        "or_adsorbingWallCL_P2": (None, [PP, P, C.c_int, P, P, P, P, P, D]),
+        "or_adsorbingWallD_P2": (None, [PP, P, C.c_int, P, P, P, P, P, D]),
     }
     for n, (r, a) in sig.items():
         f = getattr(L, n)
@@ -131,7 +136,8 @@ def make_params(consts: dict, tables) -> Params:
         setattr(p, k, int(consts[k]))
     for k in ("Pr", "Kn_eff", "gam", "B", "K", "omega", "CFL", "umax", "vmax"):
         setattr(p, k, float(consts[k]))
-    for k in ("wall_D", "wall_U", "wall_V", "wall_T", "wall_P", "beta_n", "beta_t", "gamma_f", "alpha_p", "adsorb"):
+    for k in ("wall_D", "wall_U", "wall_V", "wall_T", "wall_P", "beta_n", "beta_t", "gamma_f", "alpha_p", "adsorb",
+              "alpha_n", "alpha_t"):
         setattr(p, k, (D * 4)(*[float(x) for x in consts[k]]))
     for k in ("dynamic_vartheta", "constant_adsorb"):
         setattr(p, k, (I * 4)(*[int(x) for x in consts[k]]))

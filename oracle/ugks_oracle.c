@@ -817,8 +817,53 @@ void or_adsorbingWall_P1(const or_params *P, const od2 *normal, int face, const 
 }
 
 /* adsorbingWallS_P2 (clAdsorb.c:656-816) and adsorbingWallD_P2 (:821-975) */
+/* Modified Bessel function I0 by its integral I0(x) = (1/2pi) int_0^2pi
+ * e^{x cos t} dt under the M-point trapezoidal rule, exact up to the aliased
+ * 2 I_M(x) (< 1e-17 I0(x) for M = 256 and |x| < 800); a different route from
+ * the reference's rational fit (clFunctions.c:88-143) on purpose. */
+double or_bessel_i0(double x) {
+    const int M = 256;
+    double s = 0.0;
+    for (int k = 0; k < M; k++) s += exp(x * cos((2.0 * PI_ * k) / M));
+    return s / M;
+}
+
+/* CercignaniLampis with scatterNormal / scatterTangential,
+ * clFunctions.c:146-185: weight of incoming (normal, tangential) velocity uin
+ * in the outgoing distribution at uout, wall temperature Tw */
+double or_cl_kernel(od2 uin, od2 uout, double alpha_n, double alpha_t, double Tw) {
+    double Rn = (2.0 / (alpha_n * Tw)) * fabs(uout.x) *
+                or_bessel_i0(((2 * sqrt(1 - alpha_n)) / (alpha_n * Tw)) * uin.x * uout.x) *
+                exp(-(uout.x * uout.x + (1 - alpha_n) * uin.x * uin.x) / (alpha_n * Tw));
+    double a = uout.y - (1 - alpha_t) * uin.y;
+    double Rt = (1.0 / sqrt(PI_ * alpha_t * (2 - alpha_t) * Tw)) * exp(-(a * a) / (alpha_t * (2 - alpha_t) * Tw));
+    return fabs(uin.x / uout.x) * Rn * Rt;
+}
+
+/* The re-emitted (g, h) of outgoing velocity gv under Cercignani-Lampis:
+ * the sum over the velocities into the wall (u_n != 0) of the scattered
+ * reflected distribution WALL_DIST, clAdsorb.c:557-574 */
+static od2 cl_emission(const or_params *P, od2 n, int rot, int face, int ci, od4 wall, od2 uv, int gv,
+                       const od2 *wall_dist) {
+    double an = P->alpha_n[face], at = P->alpha_t[face];
+    od2 s = v2(0.0, 0.0);
+    for (int va = 0; va < NV_; va++) {
+        od2 uin = interfaceVelocity(P, va, n);
+        if (wall_delta(uin.x, rot) || uin.x == 0.0) continue;
+        double k = or_cl_kernel(uin, uv, an, at, 1.0 / wall.s3);
+        od2 fin = wall_dist[(size_t)ci * NV_ + va];
+        od2 c = v2(k * fin.x, k * fin.y);
+        c.y *= (1 - at) * (1 - at);
+        c.y += 0.5 * at * (2 - at) / wall.s3 * c.x;
+        s = add2(s, c);
+    }
+    return smul2(P->weight[gv], s);
+}
+
+/* adsorbingWallS_P2 / D_P2 / CL_P2 (clAdsorb.c:656-816, :821-975, :488-651);
+ * mode 0 specular, 1 diffuse, 2 Cercignani-Lampis */
 static void adsorb_P2(const or_params *P, const od2 *normal, int face, const od4 *wall_prop,
-                      const od4 *wall_cover, const od2 *wall_dist, od2 *flux_f, int diffuse) {
+                      const od4 *wall_cover, const od2 *wall_dist, od2 *flux_f, int mode) {
     int nw = wall_count(P, face);
 #pragma omp parallel for num_threads(g_threads) schedule(static)
     for (int ci = 0; ci < nw; ci++) {
@@ -837,7 +882,9 @@ static void adsorb_P2(const or_params *P, const od2 *normal, int face, const od4
                 int delta = wall_delta(uv.x, rot);
                 od2 em = v2(0.0, 0.0);
                 if (delta) {
-                    if (diffuse) {
+                    if (mode == 2) {
+                        em = cl_emission(P, n, rot, face, ci, wall, uv, gv, wall_dist);
+                    } else if (mode == 1) {
                         em = or_fM(P, wall, uv, gv);
                     } else {
                         int64_t mv = fid == WEST ? P->mirror_EW[gv] : P->mirror_NS[gv];
@@ -877,6 +924,12 @@ void or_adsorbingWallD_P2(const or_params *P, const od2 *normal, int face, const
                           const od4 *macro, double dt) {
     (void)macro; (void)dt;
     adsorb_P2(P, normal, face, wall_prop, wall_cover, wall_dist, flux_f, 1);
+}
+void or_adsorbingWallCL_P2(const or_params *P, const od2 *normal, int face, const od4 *wall_prop,
+                           const od4 *wall_cover, const od2 *wall_dist, od2 *flux_f,
+                           const od4 *macro, double dt) {
+    (void)macro; (void)dt;
+    adsorb_P2(P, normal, face, wall_prop, wall_cover, wall_dist, flux_f, 2);
 }
 
 /* updateMacro, clUGKS.c:1158-1187 */
@@ -1206,7 +1259,7 @@ static int wall_flux(or_block *b, int bci, int wall, od2 *flux, od4 *flux_macro,
         else if (b->bc[bci].reflect_type == 0)
             or_adsorbingWallS_P2(P, s->normal, wall, s->wall_prop, s->wall_cover, s->wall_dist, flux, s->macro, dt);
         else
-            return 2;   /* Cercignani-Lampis re-emission is not restated */
+            or_adsorbingWallCL_P2(P, s->normal, wall, s->wall_prop, s->wall_cover, s->wall_dist, flux, s->macro, dt);
     }
     or_wallFlux(P, s->normal, s->side_length, wall, flux, flux_macro, dt);
     or_wallMassEnergyFluxes(P, s->normal, s->side_length, s->area, wall, flux, s->wall_fluxes, dt);

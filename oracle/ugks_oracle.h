@@ -20,7 +20,8 @@
  *   - or_edgeExchange for rotated or flipped connections (ori/flip paths;
  *     the pure E-W and N-S shifts are pinned by the 2x2 split test),
  *   - or_edgeOutflow,
- *   - or_adsorbingWallS_P2 (specular re-emission; the diffuse one is pinned),
+ *   - or_adsorbingWallS_P2 (specular re-emission; the diffuse and the
+ *     Cercignani-Lampis ones are pinned),
  *   - the momentum and energy columns of or_wallMassEnergyFluxes (the mass
  *     columns are pinned by the zero-net-flux test),
  *   - the clamp branches of or_adsorbingWall_P1 beyond the mass balance and
@@ -57,6 +58,8 @@ typedef struct or_params {
     const double *quad;      /* [NV][2] */
     const double *weight;    /* [NV] */
     const int64_t *mirror_NS, *mirror_EW, *mirror_D;  /* [NV] */
+    /* Cercignani-Lampis accommodation ALPHA_N / ALPHA_T (ugks_CL.py:134-148) */
+    double alpha_n[4], alpha_t[4];
 } or_params;
 
 /* Device buffers of one block (ugks_block.py:212-255, :338-407). */
@@ -144,6 +147,11 @@ void or_adsorbingWall_P1(const or_params *P, const od2 *normal, int face, const 
 void or_adsorbingWallS_P2(const or_params *P, const od2 *normal, int face, const od4 *wall_prop,
                           const od4 *wall_cover, const od2 *wall_dist, od2 *flux_f,
                           const od4 *macro, double dt);
+void or_adsorbingWallCL_P2(const or_params *P, const od2 *normal, int face, const od4 *wall_prop,
+                           const od4 *wall_cover, const od2 *wall_dist, od2 *flux_f,
+                           const od4 *macro, double dt);
+double or_bessel_i0(double x);
+double or_cl_kernel(od2 uin, od2 uout, double alpha_n, double alpha_t, double Tw);
 void or_adsorbingWallD_P2(const or_params *P, const od2 *normal, int face, const od4 *wall_prop,
                           const od4 *wall_cover, const od2 *wall_dist, od2 *flux_f,
                           const od4 *macro, double dt);

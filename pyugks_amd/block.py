@@ -168,7 +168,8 @@ class UGKSBlock:
             if self.get_flag():
                 return True
             if bcx.reflect_type == "CL":
-                raise NotImplementedError("Cercignani-Lampis re-emission (adsorbingWallCL_P2) is not implemented")
+                self.prg.adsorbingWallCL_P2(s, self.normal_D, w, self.wall_prop_D, self.wall_cover_D,
+                                            self.wall_dist_D, flux, self.macro_D, dt)
             elif bcx.reflect_type == "D":
                 self.prg.adsorbingWallD_P2(s, self.normal_D, w, self.wall_prop_D, self.wall_cover_D,
                                            self.wall_dist_D, flux, self.macro_D, dt)

@@ -132,15 +132,18 @@ def adsorbing_params(sticking: float = 0.3, desorption: float = 0.01):
 
 
 def cfg5_channel(widths=None, height: int = 256, nc: int = 40, Kn: float = 0.05, flux_method: str = "WENO5",
-                 U_in: float = 0.5, length: float = 4.0) -> UGKSData:
+                 U_in: float = 0.5, length: float = 4.0, reflect_type: str = "D", alpha_n: float = 1.0,
+                 alpha_t: float = 1.0) -> UGKSData:
     """Adsorbing-wall channel: blocks of uneven width (total 1024) x 256 cells
     on [0, 4] x [0, 1]; inflow (rho 1, T 1) west, extrapolation east,
-    adsorbing south wall (sticking 0.3, desorption 0.01, coverage 0),
-    diffuse north wall at T = 1; initial rho 1, U (0.5, 0), T 1."""
+    adsorbing south wall (sticking 0.3, desorption 0.01, coverage 0; diffuse
+    re-emission, or specular / Cercignani-Lampis with accommodation
+    alpha_n, alpha_t), diffuse north wall at T = 1; initial rho 1, U (0.5, 0), T 1."""
     widths = channel_widths() if widths is None else list(widths)
     g = _base(flux_method, Kn)
     newton_cotes(g, nc)
     ap = adsorbing_params()
+    ap.update(reflect_type=reflect_type, alpha_n=alpha_n, alpha_t=alpha_t)
     bcs = [bcd.DiffuseBC(D=1.0, T=1.0), bcd.ExtrapolateOutBC(), bcd.AdsorbingBC(D=1.0, T=1.0, **ap),
            bcd.InflowBC(D=1.0, T=1.0)]
     fill = FlowCondition(D=1.0, U=U_in, V=0.0, T=1.0)

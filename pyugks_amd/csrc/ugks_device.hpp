@@ -40,6 +40,7 @@ struct Consts {
     double Pr, Kn, gam, K, omega, umax, vmax;
     double wall_D[4], wall_U[4], wall_V[4], wall_T[4], wall_P[4];
     double beta_n[4], beta_t[4], gamma_f[4], alpha_p[4], adsorb[4];
+    double alpha_n[4], alpha_t[4];   // Cercignani-Lampis accommodation (ALPHA_N/ALPHA_T, ugks_CL.py:134-148)
     int dyn[4], cads[4];
     const d2 *quad;
     const double *weight;
@@ -109,6 +110,77 @@ __device__ __forceinline__ d2 shakhov(const Consts &C, d4 p, d2 Q, d2 uv, d2 M) 
     double part = 2 * p.s3 * (cx * cx + cy * cy);
     return d2{s * (part - 4) * M.x, s * (part - 2) * M.y};
 }
+// Modified Bessel function I0 of the normal scattering kernel.  (The
+// reference evaluates a rational Chebyshev fit, clFunctions.c:88-143; any
+// I0 accurate to a few ulp gives the same kernel to 1e-14.)  |x| <= 25: the
+// power series sum_k (x^2/4)^k / (k!)^2, all terms positive; beyond: the
+// asymptotic series e^x / sqrt(2 pi x) sum_k [(2k-1)!!]^2 / (k! (8x)^k)
+// (Abramowitz & Stegun 9.6.10, 9.7.1), whose neglected part is O(e^-2x).
+// e^x is formed as a square so that the result overflows only where I0 does.
+__host__ __device__ inline double bessel_i0(double x) {
+    x = fabs(x);
+    double term = 1.0, sum = 1.0;
+    if (x <= 25.0) {
+        const double t = 0.25 * x * x;
+        for (int k = 1; k < 64; k++) {
+            term *= t / (double)(k * k);
+            sum += term;
+            if (term <= 1e-17 * sum) break;
+        }
+        return sum;
+    }
+    const double r = 0.125 / x;
+    for (int k = 1; k < 40; k++) {
+        const double a = (double)(2 * k - 1);
+        term *= (a * a) * r / (double)k;
+        sum += term;
+        if (term <= 1e-17 * sum) break;
+    }
+    const double e = exp(0.5 * x);
+    return ((sum / sqrt(2.0 * kPi * x)) * e) * e;
+}
+// Cercignani-Lampis scattering (clFunctions.c:146-185) for the wall at
+// temperature Tw: the normal kernel (accommodation an) of incoming normal
+// speed c_in into c_out, the tangential one (at), and their product with the
+// flux ratio |u_in / u_out|, the weight of one incoming (g, h) in the outgoing
+// distribution.  Its h part carries a further (1 - at)^2.
+__host__ __device__ inline double cl_scatter_normal(double c_in, double c_out, double an, double Tw) {
+    const double s = an * Tw;
+    return (2.0 / s) * fabs(c_out) * bessel_i0(((2.0 * sqrt(1.0 - an)) / s) * c_in * c_out) *
+           exp(-(c_out * c_out + (1.0 - an) * c_in * c_in) / s);
+}
+__host__ __device__ inline double cl_scatter_tangential(double c_in, double c_out, double at, double Tw) {
+    const double s = at * (2.0 - at) * Tw, a = c_out - (1.0 - at) * c_in;
+    return (1.0 / sqrt(kPi * s)) * exp(-(a * a) / s);
+}
+__host__ __device__ inline double cl_kernel(d2 uv_in, d2 uv_out, double an, double at, double Tw) {
+    return fabs(uv_in.x / uv_out.x) * cl_scatter_normal(uv_in.x, uv_out.x, an, Tw) *
+           cl_scatter_tangential(uv_in.y, uv_out.y, at, Tw);
+}
+
+// adsorbingWallCL_P2's re-emitted (g, h) at outgoing face-frame velocity uv,
+// index v (clAdsorb.c:557-574): the Cercignani-Lampis scattering of the
+// reflected distribution fin(va) over every velocity into the wall with
+// u_n != 0, summed in index order, times the weight of v.  The h part scatters
+// with a further (1 - at)^2 and gains at (2 - at) Tw / 2 times the g part.
+template <typename Fin>
+__device__ inline d2 cl_emission(const Consts &C, int face, d2 n, int rot, d4 wall, d2 uv, int v, Fin fin) {
+    const double an = C.alpha_n[face], at = C.alpha_t[face], Tw = 1.0 / wall.s3;
+    const double hk = (1.0 - at) * (1.0 - at), hc = 0.5 * at * (2.0 - at) / wall.s3;
+    double sg = 0.0, sh = 0.0;
+    for (int va = 0; va < C.NV; va++) {
+        const d2 uin = face_velocity(C, va, n);
+        if (leaves_wall(uin.x, rot) || uin.x == 0.0) continue;
+        const double k = cl_kernel(uin, uv, an, at, Tw);
+        const d2 f = fin(va);
+        const double cg = k * f.x;
+        sg += cg;
+        sh += k * f.y * hk + hc * cg;
+    }
+    const double w = C.weight[v];
+    return d2{w * sg, w * sh};
+}
+
 // fEQ (clFunctions.c:79-86)
 __device__ __forceinline__ d2 shakhov_eq(const Consts &C, d4 p, d2 Q, d2 uv, int v) {
     d2 M = maxwell(C, p, uv, v);

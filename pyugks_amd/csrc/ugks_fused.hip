@@ -845,8 +845,8 @@ __global__ void __launch_bounds__(kT) k_wall(const FBlock *__restrict__ blocks, 
             if (threadIdx.x == 0) B.wall_cover[ck] = mk4(vartheta + dvtheta, r_final, a_final, des);
         }
         __syncthreads();   // wd complete before the mirror lookups
-        // adsorbingWallS_P2 / D_P2 (clAdsorb.c:656-975)
-        const bool diffuse = B.reflect[face] == 1;
+        // adsorbingWallS_P2 / D_P2 / CL_P2 (clAdsorb.c:488-975): 0 specular, 1 diffuse, 2 Cercignani-Lampis
+        const int mode = B.reflect[face];
         double ratio = 1.0;
         if (reflected > 0.0) {
             double o[1] = {0.0};
@@ -857,7 +857,13 @@ __global__ void __launch_bounds__(kT) k_wall(const FBlock *__restrict__ blocks, 
                     d2 uv = face_velocity(C, v, n);
                     if (leaves_wall(uv.x, rot)) {
                         double eg, eh;
-                        if (diffuse) {
+                        if (mode == 2) {
+                            const d2 m = cl_emission(C, face, n, rot, wall, uv, v, [&](int va) {
+                                const P2<T> x = wd[va];
+                                return d2{(double)x.x, (double)x.y};
+                            });
+                            eg = m.x; eh = m.y;
+                        } else if (mode == 1) {
                             d2 m = maxwell(C, wall, uv, v);
                             eg = m.x; eh = m.y;
                         } else {
@@ -1554,9 +1560,6 @@ int ugks_step_plan_create_batch(ugks_block *const *blocks, const ugks_block_buff
             }
             if ((d.type == 0 || d.type == 5) && case_of_block && case_of_block[d.other_block] != F.cs) {
                 delete p; return ugks_rt::fail(-1, "step plan: exchange partner belongs to another case");
-            }
-            if (d.type == 8 && d.reflect_type == 2) {
-                delete p; return ugks_rt::fail(-2, "step plan: Cercignani-Lampis re-emission is not implemented");
             }
             if (d.type == 8 && !C.cads[k]) {
                 delete p; return ugks_rt::fail(-2, "step plan: tabulated adsorption isotherms are not implemented");

@@ -528,9 +528,10 @@ __global__ void k_adsorbingWall_P1(Consts C, const d2 *normal, int face, d4 *wal
         }
 }
 
-// adsorbingWallS_P2 (clAdsorb.c:656-816) / adsorbingWallD_P2 (:821-975)
+// adsorbingWallS_P2 (clAdsorb.c:656-816) / adsorbingWallD_P2 (:821-975) /
+// adsorbingWallCL_P2 (:488-651); mode 0 specular, 1 diffuse, 2 Cercignani-Lampis
 __global__ void k_adsorbingWall_P2(Consts C, const d2 *normal, int face, const d4 *wall_prop, const d4 *wall_cover,
-                                   const d2 *wall_dist, d2 *flux_f, int diffuse) {
+                                   const d2 *wall_dist, d2 *flux_f, int mode) {
     long long w = wave_id();
     if (w >= wall_count(C.ni, C.nj, face)) return;
     WallSite s = wall_site(C, face, (int)w);
@@ -548,8 +549,14 @@ __global__ void k_adsorbingWall_P2(Consts C, const d2 *normal, int face, const d
             d2 uv = face_velocity(C, v, n);
             if (leaves_wall(uv.x, s.rot)) {
                 d2 em;
-                if (diffuse) em = maxwell(C, wall, uv, v);
-                else em = wall_dist[(size_t)s.ci * C.NV + (s.fid == WEST ? C.mEW[v] : C.mNS[v])];
+                if (mode == 2) {
+                    const d2 *wrow = wall_dist + (size_t)s.ci * C.NV;
+                    em = cl_emission(C, face, n, s.rot, wall, uv, v, [&](int va) { return wrow[va]; });
+                } else if (mode == 1) {
+                    em = maxwell(C, wall, uv, v);
+                } else {
+                    em = wall_dist[(size_t)s.ci * C.NV + (s.fid == WEST ? C.mEW[v] : C.mNS[v])];
+                }
                 flux_f[fidx(C, s.gi, s.gj, v)] = em;
                 out += s.rot * uv.x * em.x;
             }
@@ -1031,13 +1038,13 @@ int ugks_adsorbingWall_P1(ugks_block *blk, void *stream, const void *normal, int
 }
 
 static int adsorb_p2(ugks_block *blk, void *stream, const void *normal, int32_t face, const void *wall_prop,
-                     const void *wall_cover, const void *wall_dist, void *flux_f, int diffuse, const char *name) {
+                     const void *wall_cover, const void *wall_dist, void *flux_f, int mode, const char *name) {
     UGKS_BLK_CHECK(blk);
     UGKS_WALL_CHECK(face);
     const Consts &C = blk->c;
     hipLaunchKernelGGL(k_adsorbingWall_P2, dim3(blocks_for_waves(wall_count(C.ni, C.nj, face))), dim3(kBlock), 0,
                        S(stream), C, (const d2 *)normal, (int)face, (const d4 *)wall_prop, (const d4 *)wall_cover,
-                       (const d2 *)wall_dist, (d2 *)flux_f, diffuse);
+                       (const d2 *)wall_dist, (d2 *)flux_f, mode);
     return ugks_rt::check_launch(name);
 }
 int ugks_adsorbingWallS_P2(ugks_block *blk, void *stream, const void *normal, int32_t face, const void *wall_prop,
@@ -1049,6 +1056,11 @@ int ugks_adsorbingWallD_P2(ugks_block *blk, void *stream, const void *normal, in
                            const void *wall_cover, const void *wall_dist, void *flux_f, const void *macro, double dt) {
     (void)macro; (void)dt;
     return adsorb_p2(blk, stream, normal, face, wall_prop, wall_cover, wall_dist, flux_f, 1, "adsorbingWallD_P2");
+}
+int ugks_adsorbingWallCL_P2(ugks_block *blk, void *stream, const void *normal, int32_t face, const void *wall_prop,
+                            const void *wall_cover, const void *wall_dist, void *flux_f, const void *macro, double dt) {
+    (void)macro; (void)dt;
+    return adsorb_p2(blk, stream, normal, face, wall_prop, wall_cover, wall_dist, flux_f, 2, "adsorbingWallCL_P2");
 }
 
 int ugks_updateMacro(ugks_block *blk, void *stream, const void *flux_macro_S, const void *flux_macro_W,

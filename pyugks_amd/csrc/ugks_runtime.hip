@@ -118,6 +118,7 @@ int ugks_block_create(const ugks_block_desc *d, ugks_block **out) {
         c.wall_T[k] = d->wall_T[k]; c.wall_P[k] = d->wall_P[k];
         c.beta_n[k] = d->beta_n[k]; c.beta_t[k] = d->beta_t[k]; c.gamma_f[k] = d->gamma_f[k];
         c.alpha_p[k] = d->alpha_p[k]; c.adsorb[k] = d->adsorb[k];
+        c.alpha_n[k] = d->alpha_n[k]; c.alpha_t[k] = d->alpha_t[k];
         c.dyn[k] = d->dynamic_vartheta[k]; c.cads[k] = d->constant_adsorb[k];
     }
     char *dp = (char *)b->dev_tables;

@@ -303,6 +303,7 @@ class UGKSData:
                  CFL=bake_e15(self.CFL), umax=bake_str(self.umax), vmax=bake_str(self.vmax))
         wD, wU, wV, wT, wP = [], [], [], [], []
         bn, bt, gf, ap, dyn, cad, ads = [], [], [], [], [], [], []
+        an, at = [], []
         for bcx in b.bc_list:
             if bcx.type_of_BC in (bcd.ADSORBING, bcd.DIFFUSE, bcd.CONSTANT, bcd.INFLOW, bcd.OUTFLOW):
                 wD.append(bake_str(bcx.D))
@@ -316,6 +317,7 @@ class UGKSData:
             if bcx.type_of_BC == bcd.ADSORBING:
                 bn.append(bake_e15(bcx.beta_n)); bt.append(bake_e15(bcx.beta_t))
                 gf.append(bake_e15(bcx.gamma_f)); ap.append(bake_e15(bcx.alpha_p))
+                an.append(bake_e15(bcx.alpha_n)); at.append(bake_e15(bcx.alpha_t))
                 dyn.append(int(bcx.dynamic))
                 if np.ndim(bcx.adsorb) != 0:
                     raise NotImplementedError("tabulated adsorption isotherms are not supported")
@@ -325,9 +327,11 @@ class UGKSData:
                 cad.append(1); ads.append(bake_g(a))
             else:
                 bn.append(-1.0); bt.append(-1.0); gf.append(-1.0); ap.append(-1.0)
+                an.append(-1.0); at.append(-1.0)
                 dyn.append(-1); cad.append(0); ads.append(0.0)
         c.update(wall_D=wD, wall_U=wU, wall_V=wV, wall_T=wT, wall_P=wP, beta_n=bn, beta_t=bt,
-                 gamma_f=gf, alpha_p=ap, dynamic_vartheta=dyn, constant_adsorb=cad, adsorb=ads)
+                 gamma_f=gf, alpha_p=ap, dynamic_vartheta=dyn, constant_adsorb=cad, adsorb=ads,
+                 alpha_n=an, alpha_t=at)
         return c
 
 

@@ -46,6 +46,7 @@ KERNELS = {
     "adsorbingWall_P1": [P, I, P, P, P, P, P, D, P],
     "adsorbingWallS_P2": [P, I, P, P, P, P, P, D],
     "adsorbingWallD_P2": [P, I, P, P, P, P, P, D],
+    "adsorbingWallCL_P2": [P, I, P, P, P, P, P, D],
     "updateMacro": [P, P, P, P, P],
     "UGKS_update": [P, P, P, P, P, P, P, D, P],
     "getResidual": [P, P],
@@ -133,6 +134,7 @@ class BlockDesc(C.Structure):
         ("quad", C.POINTER(D)), ("weight", C.POINTER(D)),
         ("mirror_NS", C.POINTER(C.c_int64)), ("mirror_EW", C.POINTER(C.c_int64)),
         ("mirror_D", C.POINTER(C.c_int64)),
+        ("alpha_n", D * 4), ("alpha_t", D * 4),
     ]
 
 
@@ -309,7 +311,8 @@ def make_desc(consts: dict, tables) -> tuple:
         setattr(d, k, int(consts[k]))
     for k in ("Pr", "Kn_eff", "gam", "B", "K", "omega", "CFL", "umax", "vmax"):
         setattr(d, k, float(consts[k]))
-    for k in ("wall_D", "wall_U", "wall_V", "wall_T", "wall_P", "beta_n", "beta_t", "gamma_f", "alpha_p", "adsorb"):
+    for k in ("wall_D", "wall_U", "wall_V", "wall_T", "wall_P", "beta_n", "beta_t", "gamma_f", "alpha_p", "adsorb",
+              "alpha_n", "alpha_t"):
         setattr(d, k, (D * 4)(*[float(x) for x in consts[k]]))
     for k in ("dynamic_vartheta", "constant_adsorb"):
         setattr(d, k, (I * 4)(*[int(x) for x in consts[k]]))

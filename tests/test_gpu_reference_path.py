@@ -28,6 +28,8 @@ def small_cases():
     out["cavity_minmod_nc"] = cases.cavity(10, ("NC", 8), "minmod", 1.0, lid=0.3, wall_T=1.1)
     out["cavity_weno_rho_perturbed"] = cases.cavity(12, ("NC", 12), "WENO5", 0.001, rho_perturb=0.01)
     out["channel_adsorbing"] = cases.cfg5_channel(widths=[6, 10, 8], height=8, nc=8, Kn=0.05)
+    out["channel_adsorbing_cl"] = cases.cfg5_channel(widths=[7, 9], height=6, nc=8, Kn=0.05, reflect_type="CL",
+                                                     alpha_n=0.7, alpha_t=0.45)
     # every remaining BC type on one block
     g = cases._base("WENO5", 0.05)
     cases.newton_cotes(g, 8)

@@ -410,6 +410,107 @@ def test_adsorbing_wall_mass_balance_and_coverage_bounds():
     assert abs(m1 / m0 - 1) < 1e-7
 
 
+# ------------------------------------------------- Cercignani-Lampis wall --
+def test_bessel_i0_matches_library():
+    """The oracle's I0 (trapezoidal rule on its integral) against scipy's."""
+    for x in np.concatenate([np.linspace(0.0, 40.0, 81), [-3.5, 55.0, 80.0]]):
+        assert abs(H.or_bessel_i0(x) / special.i0(x) - 1) < 2e-14 * max(1.0, abs(x) / 10)
+
+
+def _cl_K(cin, cout, an, at, Tw):
+    """Rn * Rt of the scattering from cin into cout (the flux ratio taken out)."""
+    return H.or_cl_kernel(O.V2(*cin), O.V2(*cout), an, at, Tw) / abs(cin[0] / cout[0])
+
+
+def test_cl_kernel_reciprocity():
+    """Cercignani-Lampis obeys reciprocity against the wall Maxwellian:
+    |u| M(c) R(c -> c') = |u'| M(c') R(-c' -> -c).  A swapped (1 - alpha) or a
+    wrong exponent breaks it."""
+    rng = np.random.default_rng(11)
+    for _ in range(200):
+        an, at, Tw = rng.uniform(0.15, 1.0), rng.uniform(0.15, 1.0), rng.uniform(0.5, 2.0)
+        cin, cout = rng.uniform(-2.5, 2.5, 2), rng.uniform(-2.5, 2.5, 2)
+        M = lambda c: math.exp(-(c[0] ** 2 + c[1] ** 2) / Tw)
+        lhs = abs(cin[0]) * M(cin) * _cl_K(cin, cout, an, at, Tw)
+        rhs = abs(cout[0]) * M(cout) * _cl_K(-cout, -cin, an, at, Tw)
+        assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+
+
+def test_cl_kernel_normalisation():
+    """Every incoming velocity is re-emitted with probability one: the kernel
+    integrates to 1 over the outgoing half space (library quadrature)."""
+    for an, at, Tw, cin in [(0.5, 0.7, 1.0, (-1.2, 0.4)), (0.9, 0.3, 1.5, (-0.3, -1.0)), (0.25, 1.0, 0.8, (-2.0, 0.0))]:
+        val, _ = integrate.dblquad(lambda ct, cn: _cl_K(cin, (cn, ct), an, at, Tw), 0.0, 14.0, -14.0, 14.0,
+                                   epsabs=1e-13, epsrel=1e-11)
+        assert abs(val - 1) < 1e-9
+
+
+def _cl_channel(reflect, an=1.0, at=1.0, k_f=0.0):
+    g = cases._base("WENO5", 0.05)
+    cases.newton_cotes(g, 12)
+    ads = bcd.AdsorbingBC(D=1.0, T=1.2, k_f=k_f, adsorb=0.5, reflect_type=reflect, alpha_n=an, alpha_t=at,
+                          S_T=1.0, cover_initial=0.0)
+    bcs = [bcd.DiffuseBC(D=1.0, T=1.0, U=0.1), bcd.DiffuseBC(D=1.0, T=1.0), ads, bcd.DiffuseBC(D=1.0, T=1.0)]
+    g.blocks = cartesian_superblock(0.0, 1.0, 0.0, 1.0, 8, 6, bcs, FlowCondition(D=1.0, U=0.05, T=1.0))
+    return g
+
+
+def test_cl_full_accommodation_is_diffuse():
+    """alpha_n = alpha_t = 1 makes the kernel independent of the incoming
+    velocity: the re-emitted gas is the wall Maxwellian, so (K = 1) the
+    Cercignani-Lampis wall steps like the diffuse one."""
+    od, oc = run(_cl_channel("D"), 3), run(_cl_channel("CL"), 3)
+    fd, fc = od.blocks[0]["f"], oc.blocks[0]["f"]
+    assert np.max(np.abs(fd - fc)) < 1e-12 * np.max(np.abs(fd))
+
+
+def test_cl_keeps_the_wall_internal_energy_ratio():
+    """With h = g Tw / 2 in every impinging velocity (internal energy at the
+    wall temperature, K = 1), the re-emitted h / g is Tw / 2 for any
+    accommodation: (1 - at)^2 + at (2 - at) = 1."""
+    o = O.OracleSim(_cl_channel("CL", 0.6, 0.35), threads=2)
+    b = o.blocks[0]
+    Tw = 1.2
+    rng = np.random.default_rng(5)
+    wd = b["wall_dist"]
+    wd[..., 0] = rng.uniform(0.1, 1.0, wd.shape[:-1])
+    wd[..., 1] = 0.5 * Tw * wd[..., 0]
+    cov = b["wall_cover"]
+    cov[:, bcd.SOUTH] = [0.1, 0.8, 0.0, 0.0]         # coverage, reflected > 0, nothing adsorbed or desorbed
+    ff = b["flux_f_S"]
+    ff[:] = 7.0
+    p = [C.c_void_p(b[k].ctypes.data) for k in ("normal", "wall_prop", "wall_cover", "wall_dist", "flux_f_S", "macro")]
+    H.or_adsorbingWallCL_P2(C.byref(b.block.p), p[0], bcd.SOUTH, p[1], p[2], p[3], p[4], p[5], 1e-3)
+    G, nv = 3, b.consts["NV"]
+    q = np.array(b.tables[0]).reshape(nv, 2)
+    out = q[:, 1] > 0                                 # leaving the south wall
+    row = ff[G:G + 8, G]
+    assert np.all(row[:, ~out] == 7.0)
+    assert np.max(np.abs(row[:, out, 1] - 0.5 * Tw * row[:, out, 0])) < 1e-13 * np.max(np.abs(row[:, out, 1]))
+    # the re-emitted normal flux equals the reflected one
+    w = np.abs(q[out, 1])
+    assert np.max(np.abs((row[:, out, 0] * w).sum(axis=1) / 0.8 - 1)) < 1e-13
+
+
+def test_cl_wall_mass_balance():
+    """A closed cavity with a sticking Cercignani-Lampis wall conserves gas
+    plus adsorbed mass to rounding (equilibrium coverage 1: no desorbed
+    Maxwellian, whose discrete flux is exact only to quadrature accuracy)."""
+    g = _cl_channel("CL", 0.8, 0.5, k_f=0.3)
+    g.blocks[0].bc_list[bcd.SOUTH].adsorb = 1.0
+    o = O.OracleSim(g, threads=4)
+    b = o.blocks[0]
+    G = 3
+    A = b["area"][G:-G, G:-G]
+    L = b["side_length"][G:-G, G, 0]
+    m0 = np.sum(A * b["macro"][:, :, 0]) + np.sum(b["wall_cover"][:8, bcd.SOUTH, 0] * L)
+    o.run(15)
+    th = b["wall_cover"][:8, bcd.SOUTH, 0]
+    m1 = np.sum(A * b["macro"][:, :, 0]) + np.sum(th * L)
+    assert np.all(th > 0) and np.all(th < 1)
+    assert abs(m1 / m0 - 1) < 1e-13
+
+
 def test_find_dt_closed_form():
     for Kn in (1e-3, 0.1):
         g = cases.cavity(8, ("NC", 12), "WENO5", Kn, lid=0.0)
