@@ -48,6 +48,13 @@ typedef struct ugks_block_desc {
     const double *weight;           /* host [NV]     (WEIGHT, ugks_CL.py:335-346) */
     const int64_t *mirror_NS, *mirror_EW, *mirror_D;  /* host [NV] (ugks_CL.py:348-395) */
     double alpha_n[4], alpha_t[4];  /* Cercignani-Lampis accommodation, -1 off adsorbing walls (ugks_CL.py:134-148) */
+    /* Tabulated Langmuir isotherms (clAdsorb.c:171-247, ugks_CL.py:160-248), for adsorbing walls
+     * with constant_adsorb 0: n_iso points (P, T, theta) as host [n_iso][3] doubles, and the n_tri
+     * triangles of the Delaunay triangulation of (P, T) as host [n_tri][3] vertex indices and
+     * [n_tri][3] neighbours (neighbour k opposite vertex k, -1 on the hull).  n_tri 0: no table. */
+    const double *iso[4];
+    const int32_t *iso_tri[4], *iso_nbr[4];
+    int32_t n_iso[4], n_tri[4];
 } ugks_block_desc;
 
 typedef struct ugks_block ugks_block;

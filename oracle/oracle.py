@@ -46,6 +46,7 @@ class Params(C.Structure):
         ("dynamic_vartheta", I * 4), ("constant_adsorb", I * 4), ("adsorb", D * 4),
         ("quad", P), ("weight", P), ("mirror_NS", P), ("mirror_EW", P), ("mirror_D", P),
         ("alpha_n", D * 4), ("alpha_t", D * 4),
+        ("iso", P * 4), ("iso_tri", P * 4), ("iso_nbr", P * 4), ("n_iso", I * 4), ("n_tri", I * 4),
     ]
 
 
@@ -118,6 +119,7 @@ def helpers():
         "or_getPrimary": (V4, [PP, V4]),
         "or_getConserved": (V4, [PP, V4]),
         "or_bessel_i0": (D, [D]),
+        "or_langmuir_table": (D, [PP, C.c_int, D, D, C.POINTER(C.c_int)]),
         "or_cl_kernel": (D, [V2, V2, D, D, D]),
         "or_adsorbingWallCL_P2": (None, [PP, P, C.c_int, P, P, P, P, P, D]),
         "or_adsorbingWallD_P2": (None, [PP, P, C.c_int, P, P, P, P, P, D]),
@@ -143,6 +145,10 @@ def make_params(consts: dict, tables) -> Params:
         setattr(p, k, (I * 4)(*[int(x) for x in consts[k]]))
     p.quad, p.weight = q.ctypes.data, w.ctypes.data
     p.mirror_NS, p.mirror_EW, p.mirror_D = mns.ctypes.data, mew.ctypes.data, md.ctypes.data
+    for k, (pts, tri, nbr) in enumerate(zip(consts["iso"], consts["iso_tri"], consts["iso_nbr"])):
+        if pts is not None:      # the arrays stay alive in consts
+            p.iso[k], p.iso_tri[k], p.iso_nbr[k] = pts.ctypes.data, tri.ctypes.data, nbr.ctypes.data
+            p.n_iso[k], p.n_tri[k] = len(pts), len(tri)
     return p
 
 

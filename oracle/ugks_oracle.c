@@ -728,8 +728,73 @@ static double stickingProbability(const or_params *P, od2 uv, double T, int face
     return normal * tangential;
 }
 
-/* adsorbingWall_P1, clAdsorb.c:253-482 (constant-coverage isotherm branch,
-   clAdsorb.c:186-231; the tabulated Delaunay isotherm is not restated) */
+/* intersect, clAdsorb.c:53-71: do segments ab and cd cross?  The reference
+   forms the determinants in double and divides in single precision. */
+static int seg_intersect(od2 a, od2 b, od2 c, od2 d) {
+    float den = (float)((d.y - c.y) * (b.x - a.x) - (d.x - c.x) * (b.y - a.y));
+    float num1 = (float)((d.x - c.x) * (a.y - c.y) - (d.y - c.y) * (a.x - c.x));
+    float num2 = (float)((b.x - a.x) * (a.y - c.y) - (b.y - a.y) * (a.x - c.x));
+    if (den == 0.0f) return 0;                /* parallel or coincident */
+    float u1 = num1 / den, u2 = num2 / den;
+    if (u1 < 0.0f || u1 > 1.0f || u2 < 0.0f || u2 > 1.0f) return 0;
+    return 1;
+}
+
+/* distance_from_segment, clAdsorb.c:18-51: distance of c from segment ab */
+static double seg_distance(od2 a, od2 b, od2 c) {
+    double rn = (c.x - a.x) * (b.x - a.x) + (c.y - a.y) * (b.y - a.y);
+    double rd = (b.x - a.x) * (b.x - a.x) + (b.y - a.y) * (b.y - a.y);
+    double r = rn / rd;
+    double s = ((a.y - c.y) * (b.x - a.x) - (a.x - c.x) * (b.y - a.y)) / rd;
+    if (r >= 0 && r <= 1) return fabs(s) * sqrt(rd);
+    double d1 = (c.x - a.x) * (c.x - a.x) + (c.y - a.y) * (c.y - a.y);
+    double d2 = (c.x - b.x) * (c.x - b.x) + (c.y - b.y) * (c.y - b.y);
+    return d1 < d2 ? sqrt(d1) : sqrt(d2);
+}
+
+/* vartheta_langmuir for a tabulated isotherm, clAdsorb.c:171-247 with
+   get_tri (:73-157) and interpolate (:159-168): walk the triangulation from
+   triangle 0 towards pt = (P, T), crossing the edge that the segment from the
+   current triangle's centroid to pt hits (edge bc, then ca, then ab), for at
+   most n_tri moves; stop inside, on an edge (1e-12), or at the hull (then
+   *outside = 1); barycentric interpolation of theta in the final triangle. */
+double or_langmuir_table(const or_params *P, int face, double Pr, double T, int *outside) {
+    const double *iso = P->iso[face];
+    const int32_t *tri = P->iso_tri[face], *nbr = P->iso_nbr[face];
+    od2 pt = v2(Pr, T);
+    int id = 0;
+    *outside = 0;
+    for (int counter = 0; counter < P->n_tri[face]; counter++) {
+        const int32_t *t = tri + 3 * id;
+        od2 a = v2(iso[3 * t[0]], iso[3 * t[0] + 1]);
+        od2 b = v2(iso[3 * t[1]], iso[3 * t[1] + 1]);
+        od2 c = v2(iso[3 * t[2]], iso[3 * t[2] + 1]);
+        od2 start = smul2(1.0 / 3.0, add2(add2(a, b), c));
+        int hz = seg_intersect(a, b, start, pt);
+        int hx = seg_intersect(b, c, start, pt);
+        int hy = seg_intersect(c, a, start, pt);
+        if (hx + hy + hz == 0) break;
+        const int32_t *nb = nbr + 3 * id;
+        int next;
+        od2 e0, e1;
+        if (hx) { next = nb[0]; e0 = b; e1 = c; }
+        else if (hy) { next = nb[1]; e0 = c; e1 = a; }
+        else { next = nb[2]; e0 = a; e1 = b; }
+        if (next == -1) { *outside = 1; break; }
+        if (seg_distance(e0, e1, pt) < 1e-12) break;
+        id = next;
+    }
+    const int32_t *t = tri + 3 * id;
+    const double *A = iso + 3 * t[0], *B = iso + 3 * t[1], *Cc = iso + 3 * t[2];
+    double den = (B[1] - Cc[1]) * (A[0] - Cc[0]) + (Cc[0] - B[0]) * (A[1] - Cc[1]);
+    double L1 = ((B[1] - Cc[1]) * (pt.x - Cc[0]) + (Cc[0] - B[0]) * (pt.y - Cc[1])) / den;
+    double L2 = ((Cc[1] - A[1]) * (pt.x - Cc[0]) + (A[0] - Cc[0]) * (pt.y - Cc[1])) / den;
+    double L3 = 1 - L1 - L2;
+    return L1 * A[2] + L2 * B[2] + L3 * Cc[2];
+}
+
+/* adsorbingWall_P1, clAdsorb.c:253-482, with the constant or the tabulated
+   isotherm (clAdsorb.c:171-247) */
 void or_adsorbingWall_P1(const or_params *P, const od2 *normal, int face, const od4 *wall_prop,
                          od4 *wall_cover, od2 *wall_dist, const od2 *flux_f, const od4 *macro,
                          double dt, double *flag) {
@@ -747,7 +812,6 @@ void or_adsorbingWall_P1(const or_params *P, const od2 *normal, int face, const 
         case GSOUTH: T = 1.0 / M2(macro, ci, gj - G_).s3; D = M2(macro, ci, gj - G_).s0; break;
         default:     T = 1.0 / M2(macro, gi - G_, ci).s3; D = M2(macro, gi - G_, ci).s0; break;
         }
-        (void)D;
         od2 n = FACEA(normal, gi, gj, fid);
         double vartheta = WP(wall_cover, face, ci).s0;
         od2 data = v2(0.0, 0.0);
@@ -786,7 +850,11 @@ void or_adsorbingWall_P1(const or_params *P, const od2 *normal, int face, const 
             }
             double veq_x, veq_y;
             if (P->constant_adsorb[face]) { veq_x = P->adsorb[face]; veq_y = 0.0; }
-            else { veq_x = NAN; veq_y = 1.0; }   /* tabulated isotherm: outside this restatement */
+            else if (P->n_tri[face] > 0) {
+                int out;
+                veq_x = or_langmuir_table(P, face, 0.5 * D * T, T, &out);
+                veq_y = (double)out;
+            } else { veq_x = NAN; veq_y = 1.0; }   /* no isotherm at all: reported as out of range */
             if (veq_y == 1) {
                 err = 1;
                 early = 1;

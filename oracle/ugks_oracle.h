@@ -25,7 +25,9 @@
  *   - the momentum and energy columns of or_wallMassEnergyFluxes (the mass
  *     columns are pinned by the zero-net-flux test),
  *   - the clamp branches of or_adsorbingWall_P1 beyond the mass balance and
- *     coverage bounds that are checked.
+ *     coverage bounds that are checked,
+ *   - the walk of or_langmuir_table at degenerate edge crossings (the walk
+ *     and interpolation are pinned inside the hull and off the table).
  */
 #ifndef UGKS_ORACLE_H
 #define UGKS_ORACLE_H
@@ -60,6 +62,11 @@ typedef struct or_params {
     const int64_t *mirror_NS, *mirror_EW, *mirror_D;  /* [NV] */
     /* Cercignani-Lampis accommodation ALPHA_N / ALPHA_T (ugks_CL.py:134-148) */
     double alpha_n[4], alpha_t[4];
+    /* tabulated isotherms ISO / TRI / NBR / N_TRI (ugks_CL.py:160-248): [n_iso][3] points
+     * (P, T, theta), [n_tri][3] vertices and neighbours; n_tri 0 where a wall has none */
+    const double *iso[4];
+    const int32_t *iso_tri[4], *iso_nbr[4];
+    int32_t n_iso[4], n_tri[4];
 } or_params;
 
 /* Device buffers of one block (ugks_block.py:212-255, :338-407). */
@@ -151,6 +158,7 @@ void or_adsorbingWallCL_P2(const or_params *P, const od2 *normal, int face, cons
                            const od4 *wall_cover, const od2 *wall_dist, od2 *flux_f,
                            const od4 *macro, double dt);
 double or_bessel_i0(double x);
+double or_langmuir_table(const or_params *P, int face, double Pr, double T, int *outside);
 double or_cl_kernel(od2 uin, od2 uout, double alpha_n, double alpha_t, double Tw);
 void or_adsorbingWallD_P2(const or_params *P, const od2 *normal, int face, const od4 *wall_prop,
                           const od4 *wall_cover, const od2 *wall_dist, od2 *flux_f,

@@ -42,6 +42,11 @@ struct Consts {
     double beta_n[4], beta_t[4], gamma_f[4], alpha_p[4], adsorb[4];
     double alpha_n[4], alpha_t[4];   // Cercignani-Lampis accommodation (ALPHA_N/ALPHA_T, ugks_CL.py:134-148)
     int dyn[4], cads[4];
+    // tabulated isotherms (ISO/TRI/NBR/N_TRI, ugks_CL.py:160-248): [.][3] points (P, T, theta),
+    // vertex and neighbour triples; n_tri 0 where a wall has none
+    const double *iso[4];
+    const int *iso_tri[4], *iso_nbr[4];
+    int n_tri[4];
     const d2 *quad;
     const double *weight;
     const int64_t *mNS, *mEW, *mD;
@@ -179,6 +184,75 @@ __device__ inline d2 cl_emission(const Consts &C, int face, d2 n, int rot, d4 wa
     }
     const double w = C.weight[v];
     return d2{w * sg, w * sh};
+}
+
+// ---- tabulated Langmuir isotherm (vartheta_langmuir, clAdsorb.c:171-247) ----
+// Does segment ab cross segment cd?  (intersect, clAdsorb.c:53-71: the
+// determinants in double, the two ratios in single precision.)  The walk's
+// decisions hang on these roundings, so nothing here is fused into an FMA.
+__host__ __device__ inline int segments_cross(d2 a, d2 b, d2 c, d2 d) {
+#pragma clang fp contract(off)
+    const float den = (float)((d.y - c.y) * (b.x - a.x) - (d.x - c.x) * (b.y - a.y));
+    if (den == 0.0f) return 0;
+    const float r1 = (float)((d.x - c.x) * (a.y - c.y) - (d.y - c.y) * (a.x - c.x)) / den;
+    const float r2 = (float)((b.x - a.x) * (a.y - c.y) - (b.y - a.y) * (a.x - c.x)) / den;
+    return (r1 >= 0.0f && r1 <= 1.0f && r2 >= 0.0f && r2 <= 1.0f) ? 1 : 0;
+}
+// distance of p from segment ab (distance_from_segment, clAdsorb.c:18-51)
+__host__ __device__ inline double segment_distance(d2 a, d2 b, d2 p) {
+#pragma clang fp contract(off)
+    const double len2 = (b.x - a.x) * (b.x - a.x) + (b.y - a.y) * (b.y - a.y);
+    const double r = ((p.x - a.x) * (b.x - a.x) + (p.y - a.y) * (b.y - a.y)) / len2;
+    if (r >= 0 && r <= 1) {
+        const double s = ((a.y - p.y) * (b.x - a.x) - (a.x - p.x) * (b.y - a.y)) / len2;
+        return fabs(s) * sqrt(len2);
+    }
+    const double da = (p.x - a.x) * (p.x - a.x) + (p.y - a.y) * (p.y - a.y);
+    const double db = (p.x - b.x) * (p.x - b.x) + (p.y - b.y) * (p.y - b.y);
+    return sqrt(da < db ? da : db);
+}
+// Equilibrium coverage at (P, T) from the wall's table: walk the Delaunay
+// triangulation from triangle 0, each move across the edge (bc, then ca,
+// then ab) that the segment from the current triangle's centroid to (P, T)
+// crosses, at most n_tri moves (get_tri, clAdsorb.c:73-157); stop inside, on
+// the crossed edge (within 1e-12), or at the hull with outside = 1; then
+// barycentric interpolation in that triangle (interpolate, :159-168).
+__device__ inline double isotherm_coverage(const Consts &C, int face, double P, double T, int &outside) {
+    const double *iso = C.iso[face];
+    const int *tri = C.iso_tri[face], *nbr = C.iso_nbr[face];
+    const d2 p = d2{P, T};
+    int t = 0;
+    outside = 0;
+    for (int moves = 0; moves < C.n_tri[face]; moves++) {
+        const int ia = tri[3 * t], ib = tri[3 * t + 1], ic = tri[3 * t + 2];
+        const d2 a = d2{iso[3 * ia], iso[3 * ia + 1]}, b = d2{iso[3 * ib], iso[3 * ib + 1]};
+        const d2 c = d2{iso[3 * ic], iso[3 * ic + 1]};
+        const d2 mid = (1.0 / 3.0) * ((a + b) + c);
+        const int cab = segments_cross(a, b, mid, p), cbc = segments_cross(b, c, mid, p);
+        const int cca = segments_cross(c, a, mid, p);
+        if (cab + cbc + cca == 0) break;
+        const int k = cbc ? 0 : (cca ? 1 : 2);           // the crossed edge is opposite vertex k
+        const d2 e0 = k == 0 ? b : (k == 1 ? c : a), e1 = k == 0 ? c : (k == 1 ? a : b);
+        const int next = nbr[3 * t + k];
+        if (next == -1) { outside = 1; break; }
+        if (segment_distance(e0, e1, p) < 1e-12) break;
+        t = next;
+    }
+    const double *A = iso + 3 * tri[3 * t], *B = iso + 3 * tri[3 * t + 1], *Cv = iso + 3 * tri[3 * t + 2];
+    const double den = (B[1] - Cv[1]) * (A[0] - Cv[0]) + (Cv[0] - B[0]) * (A[1] - Cv[1]);
+    const double l1 = ((B[1] - Cv[1]) * (P - Cv[0]) + (Cv[0] - B[0]) * (T - Cv[1])) / den;
+    const double l2 = ((Cv[1] - A[1]) * (P - Cv[0]) + (A[0] - Cv[0]) * (T - Cv[1])) / den;
+    const double l3 = 1 - l1 - l2;
+    return l1 * A[2] + l2 * B[2] + l3 * Cv[2];
+}
+// the equilibrium coverage of an adsorbing wall, given the density and
+// temperature of the cell beside it: the constant one, or the table's at
+// (D T / 2, T); outside = 1 if (P, T) lies off the table or there is none
+__device__ inline double equilibrium_coverage(const Consts &C, int face, double D, double T, int &outside) {
+    outside = 0;
+    if (C.cads[face]) return C.adsorb[face];
+    if (C.n_tri[face] <= 0) { outside = 1; return 0.0; }
+    return isotherm_coverage(C, face, 0.5 * D * T, T, outside);
 }
 
 // fEQ (clFunctions.c:79-86)

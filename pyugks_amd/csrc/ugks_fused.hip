@@ -774,7 +774,7 @@ __global__ void __launch_bounds__(kT) k_wall(const FBlock *__restrict__ blocks, 
             }
         }
     } else {
-        // adsorbingWall_P1 (clAdsorb.c:253-482), constant-coverage isotherm
+        // adsorbingWall_P1 (clAdsorb.c:253-482), constant or tabulated isotherm
         d4 cell;
         switch (face) {
         case GNORTH: cell = B.macro[(size_t)ci * C.nj + (gj - C.G - 1)]; break;
@@ -818,10 +818,12 @@ __global__ void __launch_bounds__(kT) k_wall(const FBlock *__restrict__ blocks, 
                     a_final = dvtheta / dt * C.alpha_p[face];
                 }
             }
-            if (!C.cads[face]) {
+            int outside;
+            const double veq = equilibrium_coverage(C, face, cell.s0, Tc, outside);
+            if (outside) {
                 early = true;
             } else {
-                des = gf * (1.0 / C.adsorb[face] - 1.0) * adjusted * vartheta;
+                des = gf * (1.0 / veq - 1.0) * adjusted * vartheta;
                 if (C.dyn[face] == 1) {
                     dvtheta -= dt * C.alpha_p[face] * des;
                     if ((vartheta + dvtheta) < 0.0) des = vartheta / (dt * C.alpha_p[face]);
@@ -1561,8 +1563,8 @@ int ugks_step_plan_create_batch(ugks_block *const *blocks, const ugks_block_buff
             if ((d.type == 0 || d.type == 5) && case_of_block && case_of_block[d.other_block] != F.cs) {
                 delete p; return ugks_rt::fail(-1, "step plan: exchange partner belongs to another case");
             }
-            if (d.type == 8 && !C.cads[k]) {
-                delete p; return ugks_rt::fail(-2, "step plan: tabulated adsorption isotherms are not implemented");
+            if (d.type == 8 && !C.cads[k] && C.n_tri[k] <= 0) {
+                delete p; return ugks_rt::fail(-1, "step plan: adsorbing wall without an isotherm");
             }
             wallf[k] = (d.type == 4 || d.type == 8);
         }

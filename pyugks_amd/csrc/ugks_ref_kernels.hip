@@ -451,7 +451,7 @@ __device__ __forceinline__ double sticking(const Consts &C, d2 uv, double T, int
     return nrm * tan_;
 }
 
-// adsorbingWall_P1 (clAdsorb.c:253-482), constant-coverage isotherm
+// adsorbingWall_P1 (clAdsorb.c:253-482), constant or tabulated isotherm
 __global__ void k_adsorbingWall_P1(Consts C, const d2 *normal, int face, d4 *wall_cover, d2 *wall_dist,
                                    const d2 *flux_f, const d4 *macro, double dt, double *flag) {
     long long w = wave_id();
@@ -498,10 +498,12 @@ __global__ void k_adsorbingWall_P1(Consts C, const d2 *normal, int face, d4 *wal
                 a_final = dvtheta / dt * C.alpha_p[face];
             }
         }
-        if (!C.cads[face]) {
-            early = true;   // tabulated isotherm: reported as out of range
+        int outside;
+        const double veq = equilibrium_coverage(C, face, cell.s0, T, outside);
+        if (outside) {
+            early = true;   // off the isotherm table: reported, the coverage is left as it was
         } else {
-            double gamma_b = gf * (1.0 / C.adsorb[face] - 1.0) * adjusted;
+            double gamma_b = gf * (1.0 / veq - 1.0) * adjusted;
             des = gamma_b * vartheta;
             if (C.dyn[face] == 1) {
                 dvtheta -= dt * C.alpha_p[face] * des;

@@ -76,7 +76,22 @@ int ugks_block_create(const ugks_block_desc *d, ugks_block **out) {
     for (size_t v = 0; tensor && v < nv; v++)
         tensor = d->quad[2 * v] == d->quad[2 * (v - v % nvt)] && d->quad[2 * v + 1] == d->quad[2 * (v % nvt) + 1];
     if (!tensor) nut = nvt = 0;
-    const size_t bytes = nv * (2 * sizeof(double) + sizeof(double) + 3 * sizeof(int64_t)) + (nut + nvt) * sizeof(double);
+    // tabulated isotherms: every index the device walk follows must stay inside its table
+    size_t iso_doubles = 0, iso_ints = 0;
+    for (int k = 0; k < 4; k++) {
+        const int32_t ni_ = d->n_iso[k], nt = d->n_tri[k];
+        if (nt == 0) continue;
+        if (nt < 0 || ni_ < 3 || !d->iso[k] || !d->iso_tri[k] || !d->iso_nbr[k])
+            return fail(-1, "isotherm table: need n_iso >= 3 points, triangles and neighbours");
+        for (int32_t t = 0; t < 3 * nt; t++) {
+            if (d->iso_tri[k][t] < 0 || d->iso_tri[k][t] >= ni_) return fail(-1, "isotherm table: vertex index out of range");
+            if (d->iso_nbr[k][t] < -1 || d->iso_nbr[k][t] >= nt) return fail(-1, "isotherm table: neighbour index out of range");
+        }
+        iso_doubles += 3 * (size_t)ni_;
+        iso_ints += 6 * (size_t)nt;
+    }
+    const size_t bytes = nv * (2 * sizeof(double) + sizeof(double) + 3 * sizeof(int64_t)) + (nut + nvt) * sizeof(double) +
+                         iso_doubles * sizeof(double) + iso_ints * sizeof(int32_t);
     ugks_block *b = new ugks_block();
     hipError_t e = hipGetDevice(&b->device);
     if (e != hipSuccess) { delete b; return hip_fail(e, "hipGetDevice"); }
@@ -104,6 +119,14 @@ int ugks_block_create(const ugks_block_desc *d, ugks_block **out) {
     for (size_t i = 0; i < nut; i++) ((double *)p)[i] = d->quad[2 * i * nvt];
     p += nut * sizeof(double);
     for (size_t j = 0; j < nvt; j++) ((double *)p)[j] = d->quad[2 * j + 1];
+    p += nvt * sizeof(double);
+    for (int k = 0; k < 4; k++)
+        if (d->n_tri[k] > 0) { memcpy(p, d->iso[k], 3 * (size_t)d->n_iso[k] * sizeof(double)); p += 3 * (size_t)d->n_iso[k] * sizeof(double); }
+    for (int k = 0; k < 4; k++)
+        if (d->n_tri[k] > 0) {
+            memcpy(p, d->iso_tri[k], 3 * (size_t)d->n_tri[k] * sizeof(int32_t)); p += 3 * (size_t)d->n_tri[k] * sizeof(int32_t);
+            memcpy(p, d->iso_nbr[k], 3 * (size_t)d->n_tri[k] * sizeof(int32_t)); p += 3 * (size_t)d->n_tri[k] * sizeof(int32_t);
+        }
     e = hipMemcpy(b->dev_tables, host, bytes, hipMemcpyHostToDevice);
     free(host);
     if (e != hipSuccess) { (void)hipFree(b->dev_tables); delete b; return hip_fail(e, "hipMemcpy(tables)"); }
@@ -129,7 +152,17 @@ int ugks_block_create(const ugks_block_desc *d, ugks_block **out) {
     c.mD = (const int64_t *)dp; dp += nv * sizeof(int64_t);
     c.nut = (int)nut; c.nvt = (int)nvt;
     c.qu = nut ? (const double *)dp : nullptr; dp += nut * sizeof(double);
-    c.qv = nvt ? (const double *)dp : nullptr;
+    c.qv = nvt ? (const double *)dp : nullptr; dp += nvt * sizeof(double);
+    for (int k = 0; k < 4; k++) {
+        c.n_tri[k] = d->n_tri[k];
+        c.iso[k] = nullptr; c.iso_tri[k] = c.iso_nbr[k] = nullptr;
+        if (d->n_tri[k] > 0) { c.iso[k] = (const double *)dp; dp += 3 * (size_t)d->n_iso[k] * sizeof(double); }
+    }
+    for (int k = 0; k < 4; k++)
+        if (d->n_tri[k] > 0) {
+            c.iso_tri[k] = (const int *)dp; dp += 3 * (size_t)d->n_tri[k] * sizeof(int32_t);
+            c.iso_nbr[k] = (const int *)dp; dp += 3 * (size_t)d->n_tri[k] * sizeof(int32_t);
+        }
     *out = b;
     return 0;
 }

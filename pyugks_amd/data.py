@@ -47,6 +47,21 @@ def bake_g(x: float) -> float:
     return float("%g" % float(x))
 
 
+def isotherm_tables(table) -> tuple:
+    """ISO / TRI / NBR of a tabulated Langmuir isotherm (ugks_CL.py:160-248):
+    the (P, T, theta) rows baked '%0.15e', and scipy's Delaunay triangulation
+    of the unbaked (P, T) -- vertex triples and neighbour triples, neighbour
+    k opposite vertex k and -1 on the hull."""
+    from scipy.spatial import Delaunay
+    tab = np.asarray(table, dtype=np.float64)
+    if tab.ndim != 2 or tab.shape[1] != 3:
+        raise RuntimeError("Passed in array for isotherm values is not an n*3 array")
+    d = Delaunay(tab[:, 0:2])
+    pts = np.ascontiguousarray([[bake_e15(x) for x in row] for row in tab], dtype=np.float64)
+    return (pts, np.ascontiguousarray(d.simplices, dtype=np.int32),
+            np.ascontiguousarray(d.neighbors, dtype=np.int32))
+
+
 def newton_coeff(i: int, n: int) -> float:
     """Composite Boole weights, 1-based index.  ugks_data.py:404-417
     (the end-point weight is 14/15, as the reference has it)."""
@@ -257,8 +272,12 @@ class UGKSData:
                     bc.gamma_f = bc.S_T * bc.k_f
                     if np.ndim(bc.adsorb) == 0:
                         bc.adsorb = bc.adsorb / bc.S_T
-                    else:
-                        raise NotImplementedError("tabulated adsorption isotherms are not supported")
+                    else:                      # (pressure, temperature, coverage), ugks_data.py:522-525
+                        tab = np.array(bc.adsorb, dtype=np.float64, copy=True)
+                        tab[:, 0] /= self.D_ref * self.C_ref ** 2
+                        tab[:, 1] /= self.T_ref
+                        tab[:, 2] /= bc.S_T
+                        bc.adsorb = tab
             b.x = np.asarray(b.x, dtype=np.float64) / self.L_ref
             b.y = np.asarray(b.y, dtype=np.float64) / self.L_ref
         self.dt /= self.t_ref
@@ -304,6 +323,7 @@ class UGKSData:
         wD, wU, wV, wT, wP = [], [], [], [], []
         bn, bt, gf, ap, dyn, cad, ads = [], [], [], [], [], [], []
         an, at = [], []
+        iso, iso_tri, iso_nbr = [None] * 4, [None] * 4, [None] * 4
         for bcx in b.bc_list:
             if bcx.type_of_BC in (bcd.ADSORBING, bcd.DIFFUSE, bcd.CONSTANT, bcd.INFLOW, bcd.OUTFLOW):
                 wD.append(bake_str(bcx.D))
@@ -320,18 +340,21 @@ class UGKSData:
                 an.append(bake_e15(bcx.alpha_n)); at.append(bake_e15(bcx.alpha_t))
                 dyn.append(int(bcx.dynamic))
                 if np.ndim(bcx.adsorb) != 0:
-                    raise NotImplementedError("tabulated adsorption isotherms are not supported")
-                a = float(bcx.adsorb)
-                if (a > 1.0) | (a <= 0.0):
-                    raise RuntimeError("Boundary condition equilibrium adsorption out of bounds, must be 0 < adsorb <=1")
-                cad.append(1); ads.append(bake_g(a))
+                    k = len(cad)
+                    iso[k], iso_tri[k], iso_nbr[k] = isotherm_tables(bcx.adsorb)
+                    cad.append(0); ads.append(0.0)
+                else:
+                    a = float(bcx.adsorb)
+                    if (a > 1.0) | (a <= 0.0):
+                        raise RuntimeError("Boundary condition equilibrium adsorption out of bounds, must be 0 < adsorb <=1")
+                    cad.append(1); ads.append(bake_g(a))
             else:
                 bn.append(-1.0); bt.append(-1.0); gf.append(-1.0); ap.append(-1.0)
                 an.append(-1.0); at.append(-1.0)
                 dyn.append(-1); cad.append(0); ads.append(0.0)
         c.update(wall_D=wD, wall_U=wU, wall_V=wV, wall_T=wT, wall_P=wP, beta_n=bn, beta_t=bt,
                  gamma_f=gf, alpha_p=ap, dynamic_vartheta=dyn, constant_adsorb=cad, adsorb=ads,
-                 alpha_n=an, alpha_t=at)
+                 alpha_n=an, alpha_t=at, iso=iso, iso_tri=iso_tri, iso_nbr=iso_nbr)
         return c
 
 

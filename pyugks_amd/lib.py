@@ -135,6 +135,8 @@ class BlockDesc(C.Structure):
         ("mirror_NS", C.POINTER(C.c_int64)), ("mirror_EW", C.POINTER(C.c_int64)),
         ("mirror_D", C.POINTER(C.c_int64)),
         ("alpha_n", D * 4), ("alpha_t", D * 4),
+        ("iso", C.c_void_p * 4), ("iso_tri", C.c_void_p * 4), ("iso_nbr", C.c_void_p * 4),
+        ("n_iso", I * 4), ("n_tri", I * 4),
     ]
 
 
@@ -321,7 +323,13 @@ def make_desc(consts: dict, tables) -> tuple:
     d.mirror_NS = mns.ctypes.data_as(C.POINTER(C.c_int64))
     d.mirror_EW = mew.ctypes.data_as(C.POINTER(C.c_int64))
     d.mirror_D = md.ctypes.data_as(C.POINTER(C.c_int64))
-    return d, (q, w, mns, mew, md)
+    keep = [q, w, mns, mew, md]
+    for k, (pts, tri, nbr) in enumerate(zip(consts["iso"], consts["iso_tri"], consts["iso_nbr"])):
+        if pts is not None:
+            d.iso[k], d.iso_tri[k], d.iso_nbr[k] = pts.ctypes.data, tri.ctypes.data, nbr.ctypes.data
+            d.n_iso[k], d.n_tri[k] = len(pts), len(tri)
+            keep += [pts, tri, nbr]
+    return d, tuple(keep)
 
 
 class Event:

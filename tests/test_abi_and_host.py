@@ -31,8 +31,9 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_descriptor_layouts():
-    # ugks_block_desc: 8 ints, 9 doubles, 5x4 doubles, 4x4 doubles, 2x4 ints, 4 doubles, 5 pointers, 2x4 doubles
-    expect = 8 * 4 + 9 * 8 + 20 * 8 + 16 * 8 + 8 * 4 + 4 * 8 + 5 * 8 + 8 * 8
+    # ugks_block_desc: 8 ints, 9 doubles, 5x4 doubles, 4x4 doubles, 2x4 ints, 4 doubles, 5 pointers,
+    # 2x4 doubles, 3x4 pointers, 2x4 ints
+    expect = 8 * 4 + 9 * 8 + 20 * 8 + 16 * 8 + 8 * 4 + 4 * 8 + 5 * 8 + 8 * 8 + 12 * 8 + 8 * 4
     assert C.sizeof(lib.BlockDesc) == expect
     assert C.sizeof(lib.BlockBuffers) == 21 * 8
     assert C.sizeof(lib.BCDesc) == 6 * 4

@@ -30,6 +30,14 @@ def small_cases():
     out["channel_adsorbing"] = cases.cfg5_channel(widths=[6, 10, 8], height=8, nc=8, Kn=0.05)
     out["channel_adsorbing_cl"] = cases.cfg5_channel(widths=[7, 9], height=6, nc=8, Kn=0.05, reflect_type="CL",
                                                      alpha_n=0.7, alpha_t=0.45)
+    # a tabulated Langmuir isotherm: scattered (P, T, theta) around the channel state
+    rng = np.random.default_rng(21)
+    P, T = rng.uniform(0.1, 1.0, 50), rng.uniform(0.6, 1.6, 50)
+    table = np.stack([P, T, 0.8 * P / (P + 0.3) * (1.2 - 0.25 * T)], axis=1)
+    g = cases.cfg5_channel(widths=[6, 8], height=6, nc=8, Kn=0.05)
+    for b in g.blocks:
+        b.bc_list[bcd.SOUTH].adsorb = table.copy()
+    out["channel_adsorbing_table"] = g
     # every remaining BC type on one block
     g = cases._base("WENO5", 0.05)
     cases.newton_cotes(g, 8)

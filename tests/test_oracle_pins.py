@@ -511,6 +511,60 @@ def test_cl_wall_mass_balance():
     assert abs(m1 / m0 - 1) < 1e-13
 
 
+# ------------------------------------------------- tabulated isotherm --
+def _isotherm_block(table):
+    g = cases._base("WENO5", 0.05)
+    cases.newton_cotes(g, 8)
+    ads = bcd.AdsorbingBC(D=1.0, T=1.0, k_f=0.3, adsorb=table, reflect_type="D", S_T=1.0, cover_initial=0.0)
+    bcs = [bcd.DiffuseBC(D=1.0, T=1.0), bcd.DiffuseBC(D=1.0, T=1.0), ads, bcd.DiffuseBC(D=1.0, T=1.0)]
+    g.blocks = cartesian_superblock(0.0, 1.0, 0.0, 1.0, 6, 6, bcs, FlowCondition(D=1.0, T=1.0))
+    return g
+
+
+def _scattered_isotherm(n=60, seed=4):
+    rng = np.random.default_rng(seed)
+    P = rng.uniform(0.1, 1.0, n)
+    T = rng.uniform(0.6, 1.6, n)
+    th = 0.8 * P / (P + 0.3) * (1.2 - 0.25 * T) + 0.05 * np.sin(5 * P * T)
+    return np.stack([P, T, th], axis=1)
+
+
+def test_isotherm_lookup_is_the_delaunay_interpolant():
+    """The oracle's walk + barycentric interpolation equals scipy's linear
+    interpolant on the same triangulation inside the hull, and reports
+    points off the table as outside."""
+    from scipy.interpolate import LinearNDInterpolator
+    from scipy.spatial import Delaunay
+    table = _scattered_isotherm()
+    o = O.OracleSim(_isotherm_block(table), threads=1)
+    p = o.blocks[0].block.p
+    tri = Delaunay(table[:, :2])
+    ref = LinearNDInterpolator(tri, table[:, 2])
+    rng = np.random.default_rng(9)
+    pts = np.stack([rng.uniform(0.0, 1.1, 400), rng.uniform(0.5, 1.7, 400)], axis=1)
+    inside = tri.find_simplex(pts) >= 0
+    assert inside.sum() > 200 and (~inside).sum() > 20
+    out = C.c_int()
+    for (P, T), ins in zip(pts, inside):
+        v = H.or_langmuir_table(C.byref(p), bcd.SOUTH, P, T, C.byref(out))
+        if ins:
+            assert out.value == 0
+            assert abs(v - ref(P, T)[()]) < 1e-13
+        else:
+            assert out.value == 1
+
+
+def test_flat_isotherm_table_equals_constant_isotherm():
+    """A table with theta = 0.5 at every point steps like adsorb = 0.5."""
+    table = _scattered_isotherm()
+    table[:, 2] = 0.5
+    gt, gc = _isotherm_block(table), _isotherm_block(0.5)
+    ot, oc = run(gt, 4), run(gc, 4)
+    for key in ("f", "wall_cover"):
+        a, b = ot.blocks[0][key], oc.blocks[0][key]
+        assert np.max(np.abs(a - b)) < 1e-13 * np.max(np.abs(b))
+
+
 def test_find_dt_closed_form():
     for Kn in (1e-3, 0.1):
         g = cases.cavity(8, ("NC", 12), "WENO5", Kn, lid=0.0)
