@@ -30,6 +30,11 @@ def small_cases():
     out["channel_adsorbing"] = cases.cfg5_channel(widths=[6, 10, 8], height=8, nc=8, Kn=0.05)
     out["channel_adsorbing_cl"] = cases.cfg5_channel(widths=[7, 9], height=6, nc=8, Kn=0.05, reflect_type="CL",
                                                      alpha_n=0.7, alpha_t=0.45)
+    # skewed, non-uniform cells (curved grid lines, straight walls)
+    out["curvilinear_vanleer"] = cases.curvilinear(cases.cavity(10, ("NC", 8), "vanLeer", 0.05, lid=0.2,
+                                                                n_cells_j=8))
+    out["curvilinear_weno_2x2"] = cases.curvilinear(cases.cavity(12, ("NC", 8), "WENO5", 0.01, nb=(2, 2),
+                                                                 n_cells_j=10, wall_T=1.1))
     # a tabulated Langmuir isotherm: scattered (P, T, theta) around the channel state
     rng = np.random.default_rng(21)
     P, T = rng.uniform(0.1, 1.0, 50), rng.uniform(0.6, 1.6, 50)

@@ -511,6 +511,68 @@ def test_cl_wall_mass_balance():
     assert abs(m1 / m0 - 1) < 1e-13
 
 
+# ------------------------------------------------------ curvilinear grids --
+def _curved(method="vanLeer", nb=(1, 1), lid=0.2, wall_T=1.0):
+    return cases.curvilinear(cases.cavity(10, ("NC", 8), method, 0.05, lid=lid, wall_T=wall_T, n_cells_j=8, nb=nb))
+
+
+def test_cell_geometry_on_a_curvilinear_mesh():
+    """cellGeom (clKernel.c:31-135) on skewed quadrilaterals against an
+    independent computation: area and centroid from the split into two
+    triangles, normals unit and perpendicular to their edge, mid-sides and
+    lengths from the edge end points."""
+    o = O.OracleSim(_curved(), threads=2)
+    b = o.blocks[0]
+    xy = b["xy"]
+    D, A_, B_, Cc = xy[:-1, :-1], xy[1:, :-1], xy[1:, 1:], xy[:-1, 1:]
+
+    def tri(p, q, r):
+        a = 0.5 * ((q[..., 0] - p[..., 0]) * (r[..., 1] - p[..., 1]) - (r[..., 0] - p[..., 0]) * (q[..., 1] - p[..., 1]))
+        return a, (p + q + r) / 3.0
+    a1, c1 = tri(D, A_, B_)
+    a2, c2 = tri(D, B_, Cc)
+    area = a1 + a2
+    cen = (a1[..., None] * c1 + a2[..., None] * c2) / area[..., None]
+    ok = np.isfinite(area)                 # the corner ghost cells have no vertices
+    assert ok.sum() > 150 and np.array_equal(ok, np.isfinite(b["area"]))
+    assert np.max(np.abs(b["area"] - area)[ok]) < 1e-15
+    assert np.max(np.abs(b["centre"] - cen)[ok]) < 1e-13
+    for face, (p, q) in ((0, (D, A_)), (1, (Cc, D))):
+        t = q - p
+        n = b["normal"][:, :, face]
+        assert np.max(np.abs(np.hypot(n[..., 0], n[..., 1]) - 1)[ok]) < 1e-15
+        assert np.max(np.abs(n[..., 0] * t[..., 0] + n[..., 1] * t[..., 1])[ok]) < 1e-15
+        assert np.max(np.abs(b["mid_side"][:, :, face] - 0.5 * (p + q))[ok]) < 1e-15
+        assert np.max(np.abs(b["side_length"][:, :, face] - np.hypot(t[..., 0], t[..., 1]))[ok]) < 1e-15
+
+
+def test_uniform_flow_stays_uniform_on_a_curvilinear_mesh():
+    """Free-stream preservation: a uniform moving gas in a periodic box of
+    skewed cells stays spatially uniform (it only relaxes in velocity space),
+    since each face flux is linear in its normal and the normals of a closed
+    cell sum to zero."""
+    g = cases._base("WENO5", 0.05)
+    cases.newton_cotes(g, 8)
+    bcs = [bcd.PeriodicBC(0, bcd.SOUTH), bcd.PeriodicBC(0, bcd.WEST), bcd.PeriodicBC(0, bcd.NORTH),
+           bcd.PeriodicBC(0, bcd.EAST)]
+    g.blocks = cartesian_superblock(0.0, 1.0, 0.0, 1.0, 10, 8, bcs, FlowCondition(D=1.0, U=0.2, V=-0.1, T=1.0))
+    o = O.OracleSim(cases.curvilinear(g), threads=2)
+    b = o.blocks[0]
+    o.run(5)
+    G = 3
+    f = b["f"][G:-G, G:-G]
+    assert np.max(np.abs(f - f[0, 0])) < 1e-13 * np.max(np.abs(f))
+
+
+def test_mass_is_conserved_on_a_curvilinear_mesh():
+    o = O.OracleSim(_curved("WENO5", nb=(2, 2), wall_T=1.1), threads=4)
+    G = 3
+    mass = lambda: sum(np.sum(b["area"][G:-G, G:-G] * b["macro"][:, :, 0]) for b in o.blocks)
+    m0 = mass()
+    o.run(8)
+    assert abs(mass() / m0 - 1) < 1e-13
+
+
 # ------------------------------------------------- tabulated isotherm --
 def _isotherm_block(table):
     g = cases._base("WENO5", 0.05)
