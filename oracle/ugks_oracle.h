@@ -17,8 +17,9 @@
  * force); the reference ships no golden vectors to check against.  These
  * parts have no such check and are PARITY UNPINNED (they follow a careful
  * reading of the reference only):
- *   - or_edgeExchange for rotated or flipped connections (ori/flip paths;
- *     the pure E-W and N-S shifts are pinned by the 2x2 split test),
+ *   - the velocity mirrors (flip_distribution) of or_edgeExchange (its index
+ *     maps for all sixteen face pairs, with the orientation flip, are pinned
+ *     geometrically; the pure shifts also by the 2x2 split test),
  *   - or_edgeOutflow,
  *   - or_adsorbingWallS_P2 (specular re-emission; the diffuse and the
  *     Cercignani-Lampis ones are pinned),

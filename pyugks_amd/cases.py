@@ -13,10 +13,12 @@ count.
 """
 from __future__ import annotations
 
+import copy
+
 import numpy as np
 
 from . import bc as bcd
-from .data import FlowCondition, UGKSData, cartesian_superblock
+from .data import BlockDef, FlowCondition, UGKSData, cartesian_superblock
 
 SEED = 20261202
 
@@ -87,6 +89,31 @@ def curvilinear(g: UGKSData, amp: float = 0.06) -> UGKSData:
         bump = amp * np.sin(np.pi * x) * np.sin(np.pi * y)
         b.x = x + bump * np.sin(2 * np.pi * y)
         b.y = y - bump * np.sin(2 * np.pi * x)
+    return g
+
+
+def rotated_pair_cavity(n: int, vel: tuple, flux_method: str, Kn: float, lid: float = 0.15,
+                        wall_T: float = 1.0) -> UGKSData:
+    """Lid-driven cavity on [0, 2] x [0, 1] as two n x n blocks joined by a
+    rotated connection: block 0 covers [0, 1] with i along +x and j along +y;
+    block 1 covers [1, 2] with i along +y and j along -x, so its NORTH face
+    is the interface (joined to block 0's EAST face) and its EAST face is
+    the moving lid.  Block 1 cell (i, j) is the cell (2n - 1 - j, i) of the
+    same cavity built as one 2n x n block."""
+    g = _base(flux_method, Kn)
+    kind, nv = vel
+    (gauss_hermite if kind == "GH" else newton_cotes)(g, nv)
+    h = 1.0 / n
+    k = np.arange(n + 1, dtype=np.float64) * h
+    fill = FlowCondition(D=1.0, U=0.0, V=0.0, T=1.0)
+    wall = lambda U=0.0: bcd.DiffuseBC(D=1.0, T=wall_T, U=U, V=0.0)
+    X0, Y0 = np.meshgrid(k, k, indexing="ij")
+    b0 = BlockDef(x=X0, y=Y0, bc_list=[wall(lid), bcd.AdjacentBC(1, bcd.NORTH), wall(), wall()],
+                  fill_condition=copy.deepcopy(fill), label="rot-0")
+    I, J = np.meshgrid(k, k, indexing="ij")
+    b1 = BlockDef(x=2.0 - J, y=I.copy(), bc_list=[bcd.AdjacentBC(0, bcd.EAST), wall(lid), wall(), wall()],
+                  fill_condition=copy.deepcopy(fill), label="rot-1")
+    g.blocks = [b0, b1]
     return g
 
 

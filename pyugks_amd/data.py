@@ -408,8 +408,94 @@ def ghost_coordinates(blocks: list, ghost: int) -> list:
                 src = oxy[G:ONI - G + 1, ONJ - 2 * G:ONJ - G]
                 xy[G:NI - G + 1, 0:G] = src + off if np.any(off) else src
             else:
-                raise NotImplementedError("only E-W and N-S block joins are supported")
+                _xy_exchange_rotated(xy, face, oxy, bcx.other_face, G, int(bcx.orientation))
     return out
+
+
+def _swap_index(this_face, i, j, that_face, NI, NJ, NIB, NJB, G, vertex):
+    """getSwapIndex (clGhostRoutines.c:47-138): the index in the other block of
+    ghost index (i, j) of this one (NI x NJ cells), for vertices or cells."""
+    af, ar = (0, 1) if vertex else (-1, 0)
+
+    def rot90(i, j, n, Ni, Nj):            # n quarter turns counter-clockwise (:5-23)
+        for _ in range(n):
+            i, j, Ni, Nj = Nj - j - 1, i, Nj, Ni
+        return i, j
+    N, E, S, W = bcd.NORTH, bcd.EAST, bcd.SOUTH, bcd.WEST
+    if this_face == E:
+        if that_face == E: return NI + af - i + NIB - 2 * G, j
+        if that_face == W: return i - (NI - 2 * G), j
+        if that_face == N:
+            i, j = rot90(i, j, 3, NI + ar, NJ + ar); return i, j + NJB - 2 * G
+        i, j = rot90(i, j, 1, NI + ar, NJ + ar); return i, j - (NI - 2 * G)
+    if this_face == W:
+        if that_face == W: return NI + af - i - (NI - 2 * G), j
+        if that_face == E: return i + NIB - 2 * G, j
+        if that_face == N:
+            i, j = rot90(i, j, 1, NI + ar, NJ + ar); return i, j + NJB - 2 * G
+        i, j = rot90(i, j, 3, NI + ar, NJ + ar); return i, j - (NI - 2 * G)
+    if this_face == N:
+        if that_face == N: return i, NJ + af - j + NJB - 2 * G
+        if that_face == S: return i, j - (NJ - 2 * G)
+        if that_face == E:
+            i, j = rot90(i, j, 1, NI + ar, NJ + ar); return i + NIB - 2 * G, j
+        i, j = rot90(i, j, 3, NI + ar, NJ + ar); return i - (NJ - 2 * G), j
+    if that_face == S: return i, NJ + af - j - (NJ - 2 * G)
+    if that_face == N: return i, j + NJB - 2 * G
+    if that_face == E:
+        i, j = rot90(i, j, 3, NI + ar, NJ + ar); return i + NIB - 2 * G, j
+    i, j = rot90(i, j, 1, NI + ar, NJ + ar); return i - (NJ - 2 * G), j
+
+
+def _face_ends(face, NI, NJ, G):
+    """First and second vertex of a block face (clGhostRoutines.c:171-236)."""
+    return {bcd.NORTH: ((G, NJ - G), (NI - G, NJ - G)), bcd.EAST: ((NI - G, G), (NI - G, NJ - G)),
+            bcd.SOUTH: ((G, G), (NI - G, G)), bcd.WEST: ((G, G), (G, NJ - G))}[face]
+
+
+def _xy_exchange_rotated(xy, face, oxy, other_face, G, ori):
+    """xyExchange (clGhostRoutines.c:155-296) for joins other than E-W / N-S:
+    every ghost vertex beyond this face takes the other block's vertex at the
+    vertex form of getSwapIndex, after the row flip of a misaligned, oriented
+    join.  Faces that coincide (end points equal in either order: a
+    conforming join) copy the vertices, the reference's NO_TRANSFORM; the
+    reference's default TRANSFORM maps the other face's first end point onto
+    this face's first, which for a conforming join with its end points in
+    opposite order would mirror the ghost vertices back across the join.
+    Faces apart (periodic joins) take that similarity (rotation, scale,
+    shift) as the reference does."""
+    NI, NJ = xy.shape[0] - 1, xy.shape[1] - 1
+    ONI, ONJ = oxy.shape[0] - 1, oxy.shape[1] - 1
+    ni, nj = NI - 2 * G, NJ - 2 * G
+    (a1i, a1j), (a2i, a2j) = _face_ends(face, NI, NJ, G)
+    (b1i, b1j), (b2i, b2j) = _face_ends(other_face, ONI, ONJ, G)
+    A1, A2, B1, B2 = (complex(*xy[a1i, a1j]), complex(*xy[a2i, a2j]), complex(*oxy[b1i, b1j]),
+                      complex(*oxy[b2i, b2j]))
+    misalign = abs(A1.real - B1.real) > 1e-12 or abs(A1.imag - B1.imag) > 1e-12 or \
+        abs(A2.real - B2.real) > 1e-12 or abs(A2.imag - B2.imag) > 1e-12
+    coincide = not misalign or (abs(A1 - B2) <= 1e-12 and abs(A2 - B1) <= 1e-12)
+    scale = (A2 - A1) / (B2 - B1)
+    if face in (bcd.NORTH, bcd.SOUTH):
+        gis, gjs = range(G, G + ni), (range(NJ - G + 1, NJ) if face == bcd.NORTH else range(0, G - 1))
+    else:
+        gis, gjs = (range(NI - G + 1, NI) if face == bcd.EAST else range(0, G - 1)), range(G, G + nj)
+    for gi in gis:
+        for gj in gjs:
+            for ii in (0, 1):
+                for jj in (0, 1):
+                    iB, jB = gi + ii, gj + jj
+                    if misalign and ori == 1:
+                        if face in (bcd.EAST, bcd.WEST):
+                            jB = NJ - jB
+                        else:
+                            iB = NI - iB
+                    iB, jB = _swap_index(face, iB, jB, other_face, NI, NJ, ONI, ONJ, G, True)
+                    p = oxy[iB, jB]
+                    if coincide:
+                        xy[gi + ii, gj + jj] = p
+                    else:
+                        z = A1 + (complex(*p) - B1) * scale
+                        xy[gi + ii, gj + jj] = (z.real, z.imag)
 
 
 def _xy_extrapolate(xy, face, ni, nj, G):

@@ -14,7 +14,7 @@ import pytest
 
 from helpers import BUFFERS, SCALE_FLOOR, rel_err
 from pyugks_amd import bc as bcd
-from pyugks_amd import cases
+from pyugks_amd import cases, joins
 from pyugks_amd.data import FlowCondition, UGKSData, cartesian_superblock
 
 TOL = 1e-12
@@ -35,6 +35,11 @@ def small_cases():
                                                                 n_cells_j=8))
     out["curvilinear_weno_2x2"] = cases.curvilinear(cases.cavity(12, ("NC", 8), "WENO5", 0.01, nb=(2, 2),
                                                                  n_cells_j=10, wall_T=1.1))
+    # two blocks joined through turned faces: E-N (aligned), E-E (flipped), N-W and S-E (quarter turns, held)
+    out["join_E_N"] = joins.joined_pair(bcd.EAST, 1, n=6, flux_method="WENO5")[0]
+    out["join_E_E"] = joins.joined_pair(bcd.EAST, 2, n=6, flux_method="vanLeer")[0]
+    out["join_N_W"] = joins.joined_pair(bcd.NORTH, 1, n=6, flux_method="WENO5")[0]
+    out["join_S_E"] = joins.joined_pair(bcd.SOUTH, 1, n=5, flux_method="minmod")[0]
     # a tabulated Langmuir isotherm: scattered (P, T, theta) around the channel state
     rng = np.random.default_rng(21)
     P, T = rng.uniform(0.1, 1.0, 50), rng.uniform(0.6, 1.6, 50)
