@@ -37,7 +37,7 @@ from scipy import integrate, special
 import oracle as O
 from pyugks_amd import bc as bcd
 from pyugks_amd import cases
-from pyugks_amd.data import (FlowCondition, UGKSData, bake_e15, cartesian_superblock, newton_coeff,
+from pyugks_amd.data import (BlockDef, FlowCondition, UGKSData, bake_e15, cartesian_superblock, newton_coeff,
                              py2_str)
 
 H = O.helpers()
@@ -571,6 +571,29 @@ def test_mass_is_conserved_on_a_curvilinear_mesh():
     m0 = mass()
     o.run(8)
     assert abs(mass() / m0 - 1) < 1e-13
+
+
+def test_slope_sign_follows_the_reference():
+    """iFace stores the slope along the upwind direction and distFlux uses it
+    as -Mt4 u^2 sigma for both signs of u_n (clUGKS.c:180-195, :760-763), so
+    a cavity whose grid indices run the other way (a half turn) steps
+    differently from the unturned one.  A restatement that used the slope
+    along the normal instead would make the two agree to rounding."""
+    from pyugks_amd import joins
+    g0 = cases._base("vanLeer", 0.05)
+    cases.newton_cotes(g0, 8)
+    g1 = copy.deepcopy(g0)
+    wall = lambda U=0.0: bcd.DiffuseBC(D=1.0, T=1.0, U=U, V=0.0)
+    fill = FlowCondition(D=1.0, T=1.0)
+    x0, y0 = joins.turned_grid(6, 0)
+    x1, y1 = joins.turned_grid(6, 2)
+    g0.blocks = [BlockDef(x=x0, y=y0, bc_list=[wall(0.2), wall(), wall(), wall()], fill_condition=fill)]
+    g1.blocks = [BlockDef(x=x1, y=y1, bc_list=[wall(), wall(), wall(0.2), wall()], fill_condition=copy.deepcopy(fill))]
+    o0, o1 = run(copy.deepcopy(g0), 1), run(copy.deepcopy(g1), 1)
+    # one step from the uniform start: no slopes yet, the two agree
+    assert np.max(np.abs(o0.blocks[0]["macro"] - o1.blocks[0]["macro"][::-1, ::-1])) < 1e-15
+    o0, o1 = run(g0, 2), run(g1, 2)
+    assert np.max(np.abs(o0.blocks[0]["macro"] - o1.blocks[0]["macro"][::-1, ::-1])) > 1e-6
 
 
 # ------------------------------------------------- tabulated isotherm --
