@@ -63,10 +63,20 @@ class UGKSBlock:
         self.total_area = float(np.sum(self.area_H[G:-G, G:-G]))
         self.para_D = L.DeviceArray((max(self.ni, self.nj), 4, 4), fill=0.0)
 
-    def initFunctions(self):
-        """ugks_block.py:258-432 (fresh start; restart from HDF5 is not supported)."""
+    def initFunctions(self, restart=None):
+        """ugks_block.py:258-432.  ``restart``: this block's record from
+        output.read_restart -- macro and Q from the file, and f when the file
+        holds it (else f starts as the equilibrium of that macro state)."""
         Ni, Nj, Nv, ni, nj = self.Ni, self.Nj, self.Nv, self.ni, self.nj
         self.macro_H, self.Q_H = initial_macro(self.bdef)
+        f_restart = None
+        if restart is not None:
+            self.gdata.time = restart["time"]
+            self.macro_H[:, :, 0] = restart["rho"]
+            self.macro_H[:, :, 1:3] = restart["UV"]
+            self.macro_H[:, :, 3] = 1.0 / np.asarray(restart["T"])
+            self.Q_H[:] = restart["Q"]
+            f_restart = restart.get("f")
         self.flag_H = np.zeros(2)
         self.flag_D = L.DeviceArray.from_host(self.flag_H)
         dist = (Ni, Nj, Nv, 2)
@@ -97,9 +107,12 @@ class UGKSBlock:
         self.residual_D = L.DeviceArray((ni, nj, 4), fill=0.0)
         self.bc_mdot_D = L.DeviceArray((wl, 4), fill=0.0)
         s = self.stream
-        self.prg.initFunctions(s, self.f_D, self.macro_D, self.Q_D)
-        self.prg.calcMacro(s, self.f_D, self.macro_D)
-        self.prg.calcQ(s, self.f_D, self.macro_D, self.Q_D)
+        if f_restart is not None:
+            self.f_D.upload(np.ascontiguousarray(f_restart, dtype=np.float64), s)
+        else:
+            self.prg.initFunctions(s, self.f_D, self.macro_D, self.Q_D)
+            self.prg.calcMacro(s, self.f_D, self.macro_D)
+            self.prg.calcQ(s, self.f_D, self.macro_D, self.Q_D)
         self.parametricBC(force=True)
 
     # ------------------------------------------------------------- BCs

@@ -3,9 +3,10 @@
 Follows ugks_sim.py:52-115 (initialisation order), :274-321 (time step and
 BC refresh), :407-501 (one_step, including the residual and mass monitors)
 and :503-680 (the step loop: residual cadence, stop criteria, clock limit,
-the runtime control file and the final-time clamp).  Not rebuilt: file
-output and restart, plotting, and the exec of user scripts (the runtime
-control file is read as plain ``name = literal`` assignments instead).
+the runtime control file and the final-time clamp), with restart from an
+HDF5 run (ugks_sim.py:824-838; the writer is pyugks_amd.output).  Not
+rebuilt: plotting, and the exec of user scripts (the runtime control file
+is read as plain ``name = literal`` assignments instead).
 """
 from __future__ import annotations
 
@@ -19,12 +20,17 @@ from .data import UGKSData, ghost_coordinates
 
 
 class UGKSim:
-    def __init__(self, gdata: UGKSData, stream=None, prepared: bool = False):
+    def __init__(self, gdata: UGKSData, stream=None, prepared: bool = False, restart: str | None = None):
         if not prepared:
             gdata.prepare()
         self.gdata = gdata
         self.stream = stream
         self.step = 0
+        self.restart_data = None
+        if restart:
+            from .output import read_restart
+            self.restart_data = read_restart(restart)
+            self.step = gdata.step = self.restart_data["step"]
         self.force_time = False
         self.is_open = False
         self.last_error = None
@@ -36,7 +42,7 @@ class UGKSim:
         self.check_closed_domain()
         # initFunctions, updateAllBC, calcTimeStep, updateTimeStep(limit=False) (ugks_sim.py:84-92)
         for b in self.blocks:
-            b.initFunctions()
+            b.initFunctions(self.restart_data["blocks"][b.id] if self.restart_data else None)
         self.updateAllBC()
         self.calcTimeStep()
         self.updateTimeStep(limit=False)
