@@ -945,10 +945,13 @@ __global__ void __launch_bounds__(kT) k_wall(const FBlock *__restrict__ blocks, 
 // frame.  Kept for every cell (ghosts included) by the update and the ghost
 // fill; k_cellmom recomputes them all at the start of each run.
 template <typename T>
-__device__ __forceinline__ void add_moments(const Consts &C, int v, T gx, T hx, double (&m)[4]) {
-    const d2 e = C.quad[v];
+__device__ __forceinline__ void add_moments(d2 e, T gx, T hx, double (&m)[4]) {
     const double g = (double)gx, h = (double)hx;
     m[0] += g; m[1] += e.x * g; m[2] += e.y * g; m[3] += 0.5 * ((e.x * e.x + e.y * e.y) * g + h);
+}
+template <typename T>
+__device__ __forceinline__ void add_moments(const Consts &C, int v, T gx, T hx, double (&m)[4]) {
+    add_moments<T>(quad_at(C, v), gx, hx, m);
 }
 
 template <typename T>
@@ -1037,9 +1040,19 @@ __global__ void __launch_bounds__(kT) k_update(const FBlock *__restrict__ blocks
     if (sep) { iu = lane / C.nvt; jv = lane - iu * C.nvt; diu = 64 / C.nvt; djv = 64 - diu * C.nvt; }
     bool nan_seen = false;
     double mom[4] = {0.0, 0.0, 0.0, 0.0};
+    // the velocity and weight of the next iteration are loaded one iteration
+    // ahead, and this velocity's f and face fluxes before its Maxwellians, so
+    // the loads of an iteration overlap its arithmetic instead of stalling it
+    d2 enext = lane < C.NV ? quad_at(C, lane) : d2{0.0, 0.0};
+    double wnext = lane < C.NV ? weight_at(C, lane) : 0.0;
     for (int v = lane; v < C.NV; v += 64) {
-        const d2 e = C.quad[v];
-        const double w = C.weight[v];
+        const V2<T> fo = fc[v], sS = fS[v], nS_ = fS[v + nS], wW = fW[v], eW = fW[v + nE];
+        const d2 e = enext;
+        const double w = wnext;
+        if (v + 64 < C.NV) {
+            enext = quad_at(C, v + 64);
+            wnext = weight_at(C, v + 64);
+        }
         const double ox = e.x - pold.s1, oy = e.y - pold.s2, nx = e.x - p.s1, ny = e.y - p.s2;
         const double co = ox * ox + oy * oy, cn = nx * nx + ny * ny;
         double Eo, En;
@@ -1058,7 +1071,6 @@ __global__ void __launch_bounds__(kT) k_update(const FBlock *__restrict__ blocks
         const double gN = w * (rln * En), hN = gN * hfn;
         const double shN = sn * (nx * Q.x + ny * Q.y), ptN = 2 * p.s3 * cn;
         const double feqNg = gN + shN * (ptN - 4) * gN, feqNh = hN + shN * (ptN - 2) * hN;
-        const V2<T> fo = fc[v], sS = fS[v], nS_ = fS[v + nS], wW = fW[v], eW = fW[v + nE];
         const double flg = (((double)sS.x - (double)nS_.x) + (double)wW.x - (double)eW.x) * rA;
         const double flh = (((double)sS.y - (double)nS_.y) + (double)wW.y - (double)eW.y) * rA;
         const double fg = (double)fo.x, fh = (double)fo.y;
@@ -1068,7 +1080,7 @@ __global__ void __launch_bounds__(kT) k_update(const FBlock *__restrict__ blocks
         if (isnan(tg) || isnan(th)) nan_seen = true;
         fc[v] = V2<T>{tg, th};
         // the new cell moments (the getAL / getAR inputs of the next step)
-        add_moments<T>(C, v, tg, th, mom);
+        add_moments<T>(e, tg, th, mom);
     }
     if (nan_seen) B.flag[0] = ERR_NAN;
 #pragma unroll
