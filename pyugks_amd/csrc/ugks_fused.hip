@@ -946,8 +946,17 @@ __global__ void __launch_bounds__(kT) k_wall(const FBlock *__restrict__ blocks, 
 // fill; k_cellmom recomputes them all at the start of each run.
 template <typename T>
 __device__ __forceinline__ void add_moments(d2 e, T gx, T hx, double (&m)[4]) {
+    // The update (where g and h are products) and the refresh (where they are
+    // loads) must round alike, or a step would depend on whether the moments
+    // came from the last update or from a refresh: no contraction here (it
+    // would fuse the sum into the multiply that made g), explicit FMAs only.
+#pragma clang fp contract(off)
     const double g = (double)gx, h = (double)hx;
-    m[0] += g; m[1] += e.x * g; m[2] += e.y * g; m[3] += 0.5 * ((e.x * e.x + e.y * e.y) * g + h);
+    const double c2 = fma(e.x, e.x, e.y * e.y);
+    m[0] += g;
+    m[1] = fma(e.x, g, m[1]);
+    m[2] = fma(e.y, g, m[2]);
+    m[3] = fma(0.5, fma(c2, g, h), m[3]);
 }
 template <typename T>
 __device__ __forceinline__ void add_moments(const Consts &C, int v, T gx, T hx, double (&m)[4]) {

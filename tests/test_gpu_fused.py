@@ -166,3 +166,27 @@ def test_autotune_keeps_state_and_results(hip, precision):
     for b1, b2 in zip(s1.blocks, s2.blocks):
         assert np.array_equal(b1.macro(), b2.macro())
         assert np.array_equal(p1.f64(b1), p2.f64(b2))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_steps_do_not_depend_on_call_chunking(hip, precision):
+    """Four steps in one call equal one step and then three: each call
+    rederives the cell moments from f, and they must equal, bit for bit, the
+    moments the update kept (a batch of three cavities whose wall
+    temperatures differ, where a fused multiply-add in the update's moment
+    sum once made them differ in the last bit)."""
+    from pyugks_amd.fused import StepPlan
+    from pyugks_amd.sim import UGKSim
+
+    def run(chunks):
+        sims = [UGKSim(g) for g in cases.cfg4_batch(ncases=3, sides=(8, 12), vgrids=(8, 12))]
+        plan = StepPlan(sims, precision)
+        for c in chunks:
+            plan.run(c)
+        out = [(b.macro(), plan.f64(b), b.Q_D.download()) for s in sims for b in s.blocks]
+        plan.close()
+        return out
+
+    for a, b in zip(run([4]), run([1, 3])):
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y)
