@@ -296,6 +296,26 @@ class UGKSBlock:
             q = q * (4 + self.gdata.K) / 5.0
         return q
 
+    def updateHost(self, get_f=False, get_wall_f=False):
+        """ugks_block.py:1350-1412: host copies of the macro and wall state
+        (macro_H, wall_cover_H, wall_fluxes_H, wall_prop_H) and of the heat
+        flux in external scaling (Q_H, calcQ then x(4+K)/5); returns f and/or
+        the two face-flux arrays on request, as the reference does."""
+        s = self.stream
+        self.macro_H = self.macro_D.download(s)
+        self.wall_cover_H = self.wall_cover_D.download(s)
+        self.wall_fluxes_H = self.wall_fluxes_D.download(s)
+        self.wall_prop_H = self.wall_prop_D.download(s)
+        self.Q_H = self.Q(external=True)
+        out = []
+        if get_f:
+            out.append(self.f_D.download(s))
+        if get_wall_f:
+            out += [self.flux_f_S_D.download(s), self.flux_f_W_D.download(s)]
+        if not out:
+            return None
+        return out[0] if len(out) == 1 else tuple(out)
+
     def block_mass(self):
         """ugks_block.py:1598-1623"""
         G = self.ghost

@@ -125,7 +125,28 @@ class ResidualOptions:
     res_drop: float = float("inf")
     non_linear_output: bool = False
     non_linear_output_limit: float = 1e5
+    non_linear_save: bool = False
     non_linear_dt: bool = False
+
+
+@dataclass
+class SaveOptions:
+    """ugks_data.py:34-68: when and what the step loop writes (HDF5 only; the
+    reference's VTK writer and picture output are not rebuilt).  save_name
+    names the run's files, h5Name overrides the full path of the global file;
+    compression is "gzip" (deflate) or None."""
+    save: bool = False
+    save_name: str = ""
+    h5Name: str = ""
+    save_initial_f: bool = False
+    save_final_f: bool = False
+    save_f_always: bool = False
+    save_final_flux: bool = False
+    save_final_wall_distribution: bool = False
+    compression: object = "gzip"
+    save_count: int = 1
+    initial_save_count: object = None
+    initial_save_cutoff_time: float = 0.0
 
 
 @dataclass
@@ -148,6 +169,9 @@ class UGKSData:
     exit: bool = False
     runtime_conf: str = ""
     residual_options: ResidualOptions = field(default_factory=ResidualOptions)
+    save_options: SaveOptions = field(default_factory=SaveOptions)
+    rootName: str = "ugks_run"      # directory the run's HDF5 output goes under (ugks_data.py:180)
+    restart: str = ""               # HDF5 global file of the run to restart from (ugks_data.py:183)
     L_ref: float = 1.0
     D_ref: float = 1.0
     T_ref: float = 273.0
@@ -283,6 +307,7 @@ class UGKSData:
         self.dt /= self.t_ref
         self.suggest_dt /= self.t_ref
         self.max_time /= self.t_ref
+        self.save_options.initial_save_cutoff_time /= self.t_ref
 
     def prepare(self):
         """global_preparation minus the job-script exec (ugks_data.py:586-668)."""

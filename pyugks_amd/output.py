@@ -168,7 +168,7 @@ class HDFOutput:
         grp.create_dataset("dt", float(g.dt))
         grp.create_dataset("time", float(g.time))
         macro = b.macro_D.download(s)
-        Q = b.Q_D.download(s)
+        Q = b.Q(external=True)         # updateHost: calcQ on the state, x(4+K)/5 (ugks_block.py:1376-1386)
         grp.create_dataset("rho", macro[:, :, 0], compression=c)
         grp.create_dataset("UV", macro[:, :, 1:3], compression=c)
         grp.create_dataset("Q", Q, compression=c)

@@ -2,14 +2,17 @@
 
 Follows ugks_sim.py:52-115 (initialisation order), :274-321 (time step and
 BC refresh), :407-501 (one_step, including the residual and mass monitors)
-and :503-680 (the step loop: residual cadence, stop criteria, clock limit,
-the runtime control file and the final-time clamp), with restart from an
-HDF5 run (ugks_sim.py:824-838; the writer is pyugks_amd.output).  Not
-rebuilt: plotting, and the exec of user scripts (the runtime control file
-is read as plain ``name = literal`` assignments instead).
+and :503-711 (the step loop: residual cadence, stop criteria, clock limit,
+the runtime control file, the final-time clamp and the save cadence of
+gdata.save_options), with HDF5 output and restart (ugks_sim.py:824-1117;
+the writer is pyugks_amd.output).  Not rebuilt: plotting, the OpenCL
+work-size search (the fused plan's autotuner takes its place), and the exec
+of user scripts (the runtime control file is read as plain
+``name = literal`` assignments instead).
 """
 from __future__ import annotations
 
+import os
 import time
 
 import numpy as np
@@ -21,31 +24,150 @@ from .data import UGKSData, ghost_coordinates
 
 class UGKSim:
     def __init__(self, gdata: UGKSData, stream=None, prepared: bool = False, restart: str | None = None):
+        """The initialisation order of ugks_sim.py:52-115: restart, blocks
+        and their geometry, initial distributions, ghost cells, first time
+        step, then the initial save.  ``restart`` (or gdata.restart) names the
+        global HDF5 file of a run to continue."""
         if not prepared:
             gdata.prepare()
         self.gdata = gdata
         self.stream = stream
         self.step = 0
-        self.restart_data = None
         if restart:
-            from .output import read_restart
-            self.restart_data = read_restart(restart)
-            self.step = gdata.step = self.restart_data["step"]
+            gdata.restart = restart
+        self.runtime_variables()
+        self.restart()
         self.force_time = False
         self.is_open = False
         self.last_error = None
-        xys = ghost_coordinates(gdata.blocks, gdata.ghost)
-        # load_blocks + initGeom (ugks_sim.py:239-262)
-        self.blocks = [UGKSBlock(i, gdata, b, xys[i], stream) for i, b in enumerate(gdata.blocks)]
-        for b in self.blocks:
-            b.blockList = self.blocks
+        self.load_blocks()
         self.check_closed_domain()
-        # initFunctions, updateAllBC, calcTimeStep, updateTimeStep(limit=False) (ugks_sim.py:84-92)
-        for b in self.blocks:
-            b.initFunctions(self.restart_data["blocks"][b.id] if self.restart_data else None)
+        self.initGeom()
+        self.initFunctions()
         self.updateAllBC()
         self.calcTimeStep()
         self.updateTimeStep(limit=False)
+        self.saved = False
+        self.output = None
+        if not gdata.restart:
+            self.saveToFile(save_f=gdata.save_options.save_initial_f)
+
+    def runtime_variables(self):
+        """ugks_sim.py:119-126"""
+        self.time_history_residual = []
+        self.time_history_residual_N = []
+
+    def restart(self):
+        """ugks_sim.py:824-838: the step, residual history and per-block state
+        of the run named by gdata.restart (read by pyugks_amd.output)."""
+        self.restart_data = None
+        if self.gdata.restart:
+            from .output import read_restart
+            self.restart_data = read_restart(self.gdata.restart)
+            self.step = self.gdata.step = self.restart_data["step"]
+            if self.gdata.residual_options.get_residual and "residual_N" in self.restart_data:
+                self.time_history_residual_N = list(self.restart_data["residual_N"])
+                self.time_history_residual = list(self.restart_data["residual"])
+
+    def load_blocks(self):
+        """ugks_sim.py:239-254; each block builds its geometry as it is made
+        (the ghost vertices of every block come first, as initGeom needs them)."""
+        g = self.gdata
+        xys = ghost_coordinates(g.blocks, g.ghost)
+        self.blocks = [UGKSBlock(i, g, b, xys[i], self.stream) for i, b in enumerate(g.blocks)]
+        for b in self.blocks:
+            b.blockList = self.blocks
+
+    def initGeom(self):
+        """ugks_sim.py:256-262: the blocks' geometry (cellGeom) from their
+        ghost-extended vertices; load_blocks has already run it, so this only
+        re-runs it when called again."""
+        if getattr(self, "_geom_done", False):
+            xys = ghost_coordinates(self.gdata.blocks, self.gdata.ghost)
+            for b in self.blocks:
+                b.initGeom(xys[b.id])
+        self._geom_done = True
+
+    def initFunctions(self):
+        """ugks_sim.py:264-272"""
+        for b in self.blocks:
+            b.initFunctions(self.restart_data["blocks"][b.id] if self.restart_data else None)
+
+    def start_timer(self):
+        """ugks_sim.py:171-178"""
+        self.timer = time.time()
+
+    def stop_timer(self):
+        """ugks_sim.py:180-185"""
+        return time.time() - self.timer
+
+    @staticmethod
+    def secToTime(seconds):
+        """ugks_sim.py:813-822: h:mm:ss:ms"""
+        s, ms = divmod(seconds * 1000, 1000)
+        m, s = divmod(s, 60)
+        h, m = divmod(m, 60)
+        return "%d:%02d:%02d:%02d" % (h, m, s, ms)
+
+    def updateHost(self, getF=False):
+        """ugks_sim.py:713-720"""
+        for b in self.blocks:
+            b.updateHost(getF)
+
+    def run_optimisation(self, plan=None, reps: int = 3):
+        """ugks_sim.py:352-405 times the OpenCL work-group sizes; the fused
+        step's counterpart times the face passes' workgroup sizes instead
+        (StepPlan.autotune).  Without a plan there is nothing to tune: the
+        reference-shaped kernels derive their launch shapes themselves."""
+        return plan.autotune(reps) if plan is not None else {}
+
+    # ---- HDF5 output (ugks_sim.py:842-1117) -------------------------------
+    def h5_path(self):
+        """Where initHDF puts the global file (ugks_sim.py:846-873)."""
+        g, save = self.gdata, self.gdata.save_options
+        if save.h5Name:
+            return save.h5Name
+        if g.restart:
+            from .output import restart_output_name
+            return restart_output_name(g.restart, g.time)
+        name = save.save_name or os.path.split(os.path.abspath(g.rootName))[1]
+        return os.path.join(g.rootName, "HDF5", name, name + ".h5")
+
+    def initHDF(self):
+        """ugks_sim.py:842-1020"""
+        from .output import HDFOutput
+        save = self.gdata.save_options
+        self.output = HDFOutput(self, self.h5_path(), compression=save.compression, title=save.save_name,
+                                restart_from=self.gdata.restart or None)
+        return self.output
+
+    def saveHDF5(self, save_f=False, save_flux=False, save_wall_f=False):
+        """ugks_sim.py:1022-1063; save_wall_f (the reference's diagnostic
+        wall distributions) is accepted and not written.  While a mixed
+        precision plan steps the run, f lives in FP32: it is widened into the
+        blocks' FP64 arrays first (the saved heat flux is computed from it)."""
+        if self.output is None:
+            self.initHDF()
+        plan = getattr(self, "_plan", None)
+        if plan is not None and plan.precision == "mixed":
+            for b in self.blocks:
+                b.f_D.upload(plan.f64(b), self.stream)
+        return self.output.save(save_f=save_f, save_flux=save_flux)
+
+    def close_HDF5(self):
+        """ugks_sim.py:1065-1096"""
+        if self.output is not None:
+            self.output.close(self.time_history_residual_N, self.time_history_residual)
+            self.output = None
+
+    def saveToFile(self, close_file=False, save_f=False, save_wall_flux=False, save_wall_f=False):
+        """ugks_sim.py:1098-1117: one step file per step at most (``saved``),
+        only when gdata.save_options.save is set; close_file closes the run."""
+        if self.gdata.save_options.save and not self.saved:
+            self.saveHDF5(save_f, save_wall_flux, save_wall_f)
+            self.saved = True
+        if close_file:
+            self.close_HDF5()
 
     def check_closed_domain(self):
         """ugks_sim.py:323-334"""
@@ -213,9 +335,15 @@ class UGKSim:
         below = 0
         first_res = True
         initial_res = None
+        save = g.save_options
+        self._plan = plan
+        if g.restart:
+            self.saved = True
+        if save.save and not self.saved:
+            self.saveToFile(save_f=save.save_initial_f)
+            self.saved = False
         bulk, ads = self.total_mass()
         self.initial_mass_total = bulk + ads
-        self.time_history_residual, self.time_history_residual_N = [], []
         self.exit_reason = None
         t0 = time.time()
         while self.step < max_step:
@@ -223,6 +351,8 @@ class UGKSim:
                     and (self.step < res.non_linear_output_limit):
                 mag = int(10 ** np.floor(np.log10(self.step + 1)))
                 res.residual_count = mag
+                if res.non_linear_save:
+                    save.save_count = mag
                 if res.non_linear_dt:
                     g.dt_update_count = mag
             get_res = bool(res.get_residual and ((self.step + 1) % res.residual_count == 0))
@@ -234,6 +364,7 @@ class UGKSim:
                 _, err = self.one_step(get_dt, get_res)
             else:
                 err = self._plan_step(plan, get_res)
+            self.saved = False
             self.read_runtime_conf()
             if g.exit:
                 self.exit_reason = "interrupted"
@@ -268,6 +399,12 @@ class UGKSim:
                         break
             if log and g.print_count > 0 and (not self.step % g.print_count or get_res):
                 log("step %d, t = %0.5f" % (self.step, g.time))
+            # save cadence (ugks_sim.py:652-669)
+            if save.save_count > 0 and not self.step % save.save_count and self.step <= g.max_step - 1:
+                self.saveToFile(save_f=save.save_f_always)
+            if g.time < save.initial_save_cutoff_time and save.initial_save_count \
+                    and not self.step % save.initial_save_count:
+                self.saveToFile()
             if (t_end - g.time) < g.dt:
                 g.dt = t_end - g.time
                 self.force_time = True
@@ -276,6 +413,9 @@ class UGKSim:
         if self.exit_reason is None:
             self.exit_reason = "step count reached"
         self.wall_time = time.time() - t0
+        # the final save closes the run's files (ugks_sim.py:693-694)
+        self.saveToFile(close_file=True, save_f=save.save_final_f, save_wall_flux=save.save_final_flux,
+                        save_wall_f=save.save_final_wall_distribution)
         return self.exit_reason in ("step count reached", "time limit reached", "residual stabilised",
                                     "residual reduction limit reached", "minimum residual reached")
 

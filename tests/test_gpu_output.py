@@ -48,3 +48,43 @@ def test_save_and_restart_continue_the_run(hip, tmp_path):
         ma, mb = a.macro_D.download(a.stream), b.macro_D.download(b.stream)
         assert np.max(np.abs(fa - fb)) <= 1e-13 * np.max(np.abs(fb))
         assert np.max(np.abs(ma - mb) / np.abs(mb).max(axis=(0, 1))) <= 1e-13
+
+
+@pytest.mark.parametrize("precision", [None, "mixed"])
+def test_run_saves_on_the_reference_cadence(hip, tmp_path, precision):
+    """gdata.save_options drives the step loop's output as in ugks_sim.py:
+    the initial state at construction, every save_count steps up to
+    max_step - 1, and the final state (with f) when the loop ends, which also
+    closes the run.  The files hold the device state, including f and Q (in
+    external scaling) of a mixed-precision plan, and a run restarts from
+    them at the final step."""
+    from pyugks_amd.fused import StepPlan
+    from pyugks_amd.sim import UGKSim
+    print(f"[gpu] save cadence, plan={precision}", flush=True)
+    g = cases.cavity(8, ("NC", 8), "vanLeer", 0.05, lid=0.2, n_cells_j=6)
+    g.max_step = 5
+    g.rootName = str(tmp_path / "run")
+    g.save_options.save = True
+    g.save_options.save_count = 2
+    g.save_options.save_final_f = True
+    g.save_options.compression = None
+    fresh = copy.deepcopy(g)
+    sim = UGKSim(g)
+    plan = StepPlan(sim, precision) if precision else None
+    assert sim.run(plan=plan, log=None)
+    assert sim.step == 5 and sim.output is None
+    glob = tmp_path / "run" / "HDF5" / "run" / "run.h5"
+    with h5.File(glob) as f:
+        assert f["final_step"] == 5
+        assert list(f["sub_files"][:, 0]) == ["run_step=%03d.h5" % k for k in (0, 2, 4, 5)]
+    with h5.File(tmp_path / "run" / "HDF5" / "run" / "run_step=005.h5") as d:
+        blk = d["block_0"]
+        b = sim.blocks[0]
+        f64 = plan.f64(b) if plan is not None else b.f_D.download(b.stream)
+        assert np.array_equal(blk["f"], f64)
+        assert np.array_equal(blk["Q"], b.Q(external=True))
+        blk.close()
+    if plan is not None:
+        plan.close()
+    rs = UGKSim(fresh, restart=str(glob))
+    assert rs.step == 5 and rs.gdata.time == sim.gdata.time

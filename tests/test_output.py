@@ -56,6 +56,7 @@ class _Block:
 
     def __init__(self, bid, gdata, bdef, rng, xy):
         self.id, self.bdef, self.stream = bid, bdef, None
+        self.K = gdata.K
         self.ghost = G = gdata.ghost
         self.ni, self.nj = bdef.ni, bdef.nj
         self.Ni, self.Nj = self.ni + 2 * G, self.nj + 2 * G
@@ -72,6 +73,9 @@ class _Block:
         dist = (self.Ni, self.Nj, gdata.Nv, 2)
         self.f_D = _Host(rng.uniform(size=dist))
         self.flux_f_S_D, self.flux_f_W_D = _Host(rng.standard_normal(dist)), _Host(rng.standard_normal(dist))
+
+    def Q(self, external=False):
+        return self.Q_D.download() * ((4 + self.K) / 5.0 if external else 1.0)
 
 
 class _Sim:
@@ -108,6 +112,8 @@ def test_writer_layout_xdmf_and_restart_read(tmp_path):
         m = blk.macro_D.a
         assert d["step"] == 6 and b["time"] == 0.25
         assert np.array_equal(b["rho"], m[..., 0]) and np.array_equal(b["UV"], m[..., 1:3])
+        # the heat flux is saved in external scaling, as updateHost leaves it
+        assert np.array_equal(b["Q"], blk.Q(external=True))
         assert np.array_equal(b["T"], 1.0 / m[..., 3]) and np.array_equal(b["f"], blk.f_D.a)
         G = blk.ghost
         assert np.array_equal(b["wall"]["S"]["flux"], blk.flux_f_S_D.a[G:blk.Ni - G, G])
