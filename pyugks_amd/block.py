@@ -47,12 +47,17 @@ class UGKSBlock:
         """ugks_block.py:209-255; cellGeom runs on the device (:748-751)."""
         Ni, Nj = self.Ni, self.Nj
         self.xy_H = np.ascontiguousarray(xy, dtype=np.float64)
-        self.xy_D = L.DeviceArray.from_host(self.xy_H)
-        self.centre_D = L.DeviceArray((Ni, Nj, 2), fill=-1.0)
-        self.side_D = L.DeviceArray((Ni, Nj, 2, 2), fill=-1.0)
-        self.area_D = L.DeviceArray((Ni, Nj), fill=-1.0)
-        self.normal_D = L.DeviceArray((Ni, Nj, 2, 2), fill=-1.0)
-        self.length_D = L.DeviceArray((Ni, Nj, 2), fill=-1.0)
+        if getattr(self, "xy_D", None) is not None:
+            # again (updateGeom): in place, so plans holding these buffers stay valid
+            self.xy_D.upload(self.xy_H, self.stream)
+        else:
+            self.xy_D = L.DeviceArray.from_host(self.xy_H)
+            self.centre_D = L.DeviceArray((Ni, Nj, 2), fill=-1.0)
+            self.side_D = L.DeviceArray((Ni, Nj, 2, 2), fill=-1.0)
+            self.area_D = L.DeviceArray((Ni, Nj), fill=-1.0)
+            self.normal_D = L.DeviceArray((Ni, Nj, 2, 2), fill=-1.0)
+            self.length_D = L.DeviceArray((Ni, Nj, 2), fill=-1.0)
+            self.para_D = L.DeviceArray((max(self.ni, self.nj), 4, 4), fill=0.0)
         self.prg.cellGeom(self.stream, self.xy_D, self.area_D, self.centre_D, self.side_D,
                           self.normal_D, self.length_D)
         self.area_H = self.area_D.download(self.stream)
@@ -61,7 +66,6 @@ class UGKSBlock:
         self.face_lengths = [length_H[G:-G, -G, 0], length_H[-G, G:-G, 1],
                              length_H[G:-G, G, 0], length_H[G, G:-G, 1]]
         self.total_area = float(np.sum(self.area_H[G:-G, G:-G]))
-        self.para_D = L.DeviceArray((max(self.ni, self.nj), 4, 4), fill=0.0)
 
     def initFunctions(self, restart=None):
         """ugks_block.py:258-432.  ``restart``: this block's record from
@@ -127,29 +131,60 @@ class UGKSBlock:
         self.prg.edgeExchange(self.stream, self.f_D, self.xy_D, np.int32(this_face), o.f_D, o.xy_D,
                               np.int32(o.Ni), np.int32(o.Nj), np.int32(other_face), np.int32(ori), np.int32(flip))
 
+    # ghost fills of one face (ugks_block.py:475-601)
+    def ghostExtrapolate(self, this_face):
+        """ugks_block.py:475-492"""
+        self.prg.edgeExtrapolate(self.stream, self.f_D, np.int32(this_face))
+
+    def ghostInflow(self, this_face):
+        """ugks_block.py:494-516"""
+        self.prg.edgeInflow(self.stream, self.f_D, np.int32(this_face), self.normal_D, self.macro_D, self.Q_D,
+                            self.wall_prop_D)
+
+    def ghostOutflow(self, this_face):
+        """ugks_block.py:518-541"""
+        self.prg.edgeOutflow(self.stream, self.f_D, np.int32(this_face), self.normal_D, self.macro_D, self.Q_D,
+                             self.wall_prop_D)
+
+    def ghostConstant(self, this_face):
+        """ugks_block.py:543-561"""
+        self.prg.edgeConstant(self.stream, self.f_D, np.int32(this_face), self.wall_prop_D)
+
+    def ghostMirror(self, this_face, bb=False):
+        """ugks_block.py:563-582: mirror (or, with bb, bounce-back) ghost cells"""
+        fn = self.prg.edgeBounceBack if bb else self.prg.edgeMirror
+        fn(self.stream, self.f_D, np.int32(this_face))
+
+    def edgeConstGrad(self, this_face):
+        """ugks_block.py:584-601"""
+        self.prg.edgeConstGrad(self.stream, self.f_D, np.int32(this_face))
+
     def updateBC(self):
         """ugks_block.py:621-665"""
-        s = self.stream
         for this_face, b in enumerate(self.bc_list):
             t = b.type_of_BC
-            f = np.int32(this_face)
             if t in (bcd.ADJACENT, bcd.PERIODIC):
                 self.ghostExchange(this_face, b.other_block, b.other_face, b.orientation, b.flip_distribution)
             elif t == bcd.EXTRAPOLATE_OUT:
-                self.prg.edgeExtrapolate(s, self.f_D, f)
+                self.ghostExtrapolate(this_face)
             elif t == bcd.CONSTANT:
-                self.prg.edgeConstant(s, self.f_D, f, self.wall_prop_D)
+                self.ghostConstant(this_face)
             elif t == bcd.REFLECT:
-                self.prg.edgeMirror(s, self.f_D, f)
+                self.ghostMirror(this_face)
             elif t == bcd.BOUNCE_BACK:
-                self.prg.edgeBounceBack(s, self.f_D, f)
+                self.ghostMirror(this_face, bb=True)
             elif t in (bcd.DIFFUSE, bcd.ADSORBING):
-                self.prg.edgeConstGrad(s, self.f_D, f)
+                self.edgeConstGrad(this_face)
                 self.has_accommodating = True
             elif t == bcd.INFLOW:
-                self.prg.edgeInflow(s, self.f_D, f, self.normal_D, self.macro_D, self.Q_D, self.wall_prop_D)
+                self.ghostInflow(this_face)
             elif t == bcd.OUTFLOW:
-                self.prg.edgeOutflow(s, self.f_D, f, self.normal_D, self.macro_D, self.Q_D, self.wall_prop_D)
+                self.ghostOutflow(this_face)
+
+    def updateGeom(self):
+        """ugks_block.py:734-751: the geometry of the block from its
+        ghost-extended vertices (cellGeom on the device)."""
+        self.initGeom(self.xy_H)
 
     # ------------------------------------------------------------- flags
     def get_flag(self):
@@ -296,16 +331,21 @@ class UGKSBlock:
             q = q * (4 + self.gdata.K) / 5.0
         return q
 
+    def updateHostMacro(self):
+        """ugks_block.py:1350-1363: host copies of the macro and wall state"""
+        s = self.stream
+        self.macro_H = self.macro_D.download(s)
+        self.wall_cover_H = self.wall_cover_D.download(s)
+        self.wall_fluxes_H = self.wall_fluxes_D.download(s)
+        self.wall_prop_H = self.wall_prop_D.download(s)
+
     def updateHost(self, get_f=False, get_wall_f=False):
         """ugks_block.py:1350-1412: host copies of the macro and wall state
         (macro_H, wall_cover_H, wall_fluxes_H, wall_prop_H) and of the heat
         flux in external scaling (Q_H, calcQ then x(4+K)/5); returns f and/or
         the two face-flux arrays on request, as the reference does."""
         s = self.stream
-        self.macro_H = self.macro_D.download(s)
-        self.wall_cover_H = self.wall_cover_D.download(s)
-        self.wall_fluxes_H = self.wall_fluxes_D.download(s)
-        self.wall_prop_H = self.wall_prop_D.download(s)
+        self.updateHostMacro()
         self.Q_H = self.Q(external=True)
         out = []
         if get_f:
