@@ -128,27 +128,33 @@ __device__ __forceinline__ float rcp_T(float x) { return __builtin_amdgcn_rcpf(x
 //               f_2 - s2 = 2 d3/3 - d4/6
 // with eps scaled by 12 alongside (the weights only see the ratios), so that
 // (f_L - s2) is formed without cancelling against s2.  The nonlinear weights
-// a_k = d_k / (eps + b_k)^2 enter only through sum(a_k (f_k - s2)) /
-// sum(a_k): in double both sums are scaled by prod_k (eps + b_k)^2, leaving
-// one quotient per component, and the two quotients share one reciprocal.
-// In float that product can underflow, so the weights use reciprocal
-// estimates instead.
+// a_k = d_k / (eps + b_k)^2 are written 1 / e_k^2 with e_k = 12 (eps + b_k)
+// / sqrt(d_k): the linear weights d = (0.1, 0.6, 0.3) ride in the constants
+// of e_k, which is one multiply and two FMAs from the differences.  They
+// enter only through sum(a_k (f_k - s2)) / sum(a_k): in double both sums are
+// scaled by prod_k e_k^2, leaving one quotient per component, and the two
+// quotients share one reciprocal.  In float that product can underflow, so
+// the weights use reciprocal estimates instead.
+// e_k = A_k u_k^2 + B_k v_k^2 + E_k with A_k = 13 / sqrt(d_k), B_k = 3 / sqrt(d_k), E_k = 1.2e-15 / sqrt(d_k)
+constexpr double kWA0 = 41.10960958218893, kWB0 = 9.486832980505138, kWE0 = 3.794733192202055e-15;
+constexpr double kWA1 = 16.782927833565473, kWB1 = 3.8729833462074166, kWE1 = 1.5491933384829667e-15;
+constexpr double kWA2 = 23.7346441585572, kWB2 = 5.477225575051661, kWE2 = 2.1908902300206646e-15;
 struct WenoParts { double num, den; };
 __device__ __forceinline__ WenoParts weno5_parts(double s0, double s1, double s2, double s3, double s4) {
     const double d1 = s1 - s0, d2 = s2 - s1, d3 = s3 - s2, d4 = s4 - s3;
     const double u0 = d2 - d1, v0 = 3.0 * d2 - d1;
     const double u1 = d3 - d2, v1 = d2 + d3;
     const double u2 = d4 - d3, v2 = 3.0 * d3 - d4;
-    const double e0 = 1.2e-15 + (13.0 * u0 * u0 + 3.0 * v0 * v0);
-    const double e1 = 1.2e-15 + (13.0 * u1 * u1 + 3.0 * v1 * v1);
-    const double e2 = 1.2e-15 + (13.0 * u2 * u2 + 3.0 * v2 * v2);
+    const double e0 = fma(kWA0 * u0, u0, fma(kWB0 * v0, v0, kWE0));
+    const double e1 = fma(kWA1 * u1, u1, fma(kWB1 * v1, v1, kWE1));
+    const double e2 = fma(kWA2 * u2, u2, fma(kWB2 * v2, v2, kWE2));
     const double q0 = e0 * e0, q1 = e1 * e1, q2 = e2 * e2;
     const double p0 = q1 * q2, p1 = q0 * q2, p2 = q0 * q1;
-    // d_k (f_k - s2) with d = (0.1, 0.6, 0.3)
-    const double c0 = (-0.1 / 3.0) * d1 + (0.5 / 6.0) * d2;
-    const double c1 = 0.1 * d2 + 0.2 * d3;
-    const double c2 = 0.2 * d3 - 0.05 * d4;
-    return WenoParts{p0 * c0 + p1 * c1 + p2 * c2, 0.1 * p0 + 0.6 * p1 + 0.3 * p2};
+    // (f_k - s2)
+    const double c0 = (-1.0 / 3.0) * d1 + (5.0 / 6.0) * d2;
+    const double c1 = (1.0 / 6.0) * d2 + (1.0 / 3.0) * d3;
+    const double c2 = (2.0 / 3.0) * d3 - (1.0 / 6.0) * d4;
+    return WenoParts{p0 * c0 + p1 * c1 + p2 * c2, (p0 + p1) + p2};
 }
 // the two slopes (f_L - s2) / dist of a (g, h) stencil
 __device__ __forceinline__ void weno5_pair(P2<double> s0, P2<double> s1, P2<double> s2, P2<double> s3, P2<double> s4,
@@ -156,7 +162,7 @@ __device__ __forceinline__ void weno5_pair(P2<double> s0, P2<double> s1, P2<doub
     const WenoParts a = weno5_parts(s0.x, s1.x, s2.x, s3.x, s4.x);
     const WenoParts b = weno5_parts(s0.y, s1.y, s2.y, s3.y, s4.y);
     // 1 / (den_a den_b) from the hardware estimate and two Newton steps; the
-    // dens are >= 0.1 (1.2e-15)^4 and far below overflow, so is their product
+    // dens are >= (1.2e-15)^4 and far below overflow, so is their product
     const double D = a.den * b.den;
     double r = __builtin_amdgcn_rcp(D);
     double e = fma(-D, r, 1.0);
@@ -175,14 +181,14 @@ __device__ __forceinline__ float weno5_f(float s0, float s1, float s2, float s3,
     const float u0 = d2 - d1, v0 = 3.f * d2 - d1;
     const float u1 = d3 - d2, v1 = d2 + d3;
     const float u2 = d4 - d3, v2 = 3.f * d3 - d4;
-    const float e0 = 1.2e-15f + (13.f * u0 * u0 + 3.f * v0 * v0);
-    const float e1 = 1.2e-15f + (13.f * u1 * u1 + 3.f * v1 * v1);
-    const float e2 = 1.2e-15f + (13.f * u2 * u2 + 3.f * v2 * v2);
-    const float a0 = 0.1f * rcp_T(e0 * e0), a1 = 0.6f * rcp_T(e1 * e1), a2 = 0.3f * rcp_T(e2 * e2);
+    const float e0 = fmaf((float)kWA0 * u0, u0, fmaf((float)kWB0 * v0, v0, (float)kWE0));
+    const float e1 = fmaf((float)kWA1 * u1, u1, fmaf((float)kWB1 * v1, v1, (float)kWE1));
+    const float e2 = fmaf((float)kWA2 * u2, u2, fmaf((float)kWB2 * v2, v2, (float)kWE2));
+    const float a0 = rcp_T(e0 * e0), a1 = rcp_T(e1 * e1), a2 = rcp_T(e2 * e2);
     const float c0 = (-1.f / 3.f) * d1 + (5.f / 6.f) * d2;
     const float c1 = (1.f / 6.f) * d2 + (1.f / 3.f) * d3;
     const float c2 = (2.f / 3.f) * d3 - (1.f / 6.f) * d4;
-    return (a0 * c0 + a1 * c1 + a2 * c2) * rcp_T(a0 + a1 + a2);
+    return (a0 * c0 + a1 * c1 + a2 * c2) * rcp_T((a0 + a1) + a2);
 }
 __device__ __forceinline__ void weno5_pair(P2<float> s0, P2<float> s1, P2<float> s2, P2<float> s3, P2<float> s4,
                                            float rdx, float &sgx, float &sgy) {
