@@ -567,15 +567,31 @@ __global__ void __launch_bounds__(kT) k_faceS(const FBlock *__restrict__ blocks,
                              (S[9] - 2.0 * U * S[5] + U * U * S[2] - V * S[3] + 2.0 * U * V * S[1] - V * U * U * S[0]) +
                              (S[12] - V * S[10]) / C.K);
     const d2 n = B.normal[own * 2 + L.face];
+    // The per-velocity flux (distFlux) with its per-face factors folded, so the
+    // flux pass evaluates, for P_a = a0 + a1 u + a2 w + a3 |u|^2 / 2,
+    //   Q_side = m1 (P_side + hk a3 / 2)        (side = L or R, by the upwind choice)
+    //   Q_T    = m2 (P_T + hk aT3 / 2) + m0
+    //   D_a    = m1 a3 (Mxi2 - hk^2) / 2        (D_T likewise with m2, aT)
+    //   S      = m0 0.8 (1 - Pr) lam^2 / rho ((c - U) . q)
+    //   g: Lf u (f0 (Q_T + u Q_side + S (2 lam c^2 - 4)) + m3 f - m4 u sigma)
+    //   h: Lf u (f0 (hk (Q_T + u Q_side + S (2 lam c^2 - 2)) + u D_side + D_T) + m3 f_h - m4 u sigma_h)
+    // with Lf the face length carried in f0 = w Lf rho lam / pi e^{-lam c^2} and
+    // in m3, m4; the macro flux sums take the same factors.
+    const double Lf = B.side_length[own * 2 + L.face];
+    const double lam = prim.s3, hk = C.K / (2.0 * lam), dk = m.Mxi[2] - hk * hk;
+    const double hL = 0.5 * aL.s3, hR = 0.5 * aR.s3, hT = 0.5 * aT.s3;
+    const double sm = mt.m0 * (0.8 * (1 - C.Pr) * (lam * lam) / prim.s0);
     double *o = fstate + fid * kFT;
-    o[0] = prim.s0; o[1] = prim.s1; o[2] = prim.s2; o[3] = prim.s3;
-    o[4] = aL.s0; o[5] = aL.s1; o[6] = aL.s2; o[7] = aL.s3;
-    o[8] = aR.s0; o[9] = aR.s1; o[10] = aR.s2; o[11] = aR.s3;
-    o[12] = aT.s0; o[13] = aT.s1; o[14] = aT.s2; o[15] = aT.s3;
-    o[16] = mt.m0; o[17] = mt.m1; o[18] = mt.m2; o[19] = mt.m3; o[20] = mt.m4;
-    o[21] = m.Mxi[2]; o[22] = qx; o[23] = qy;
+    o[0] = prim.s1; o[1] = prim.s2; o[2] = lam; o[3] = hk;
+    o[4] = mt.m1 * (aL.s0 + hL * hk); o[5] = mt.m1 * aL.s1; o[6] = mt.m1 * aL.s2; o[7] = mt.m1 * hL;
+    o[8] = mt.m1 * (aR.s0 + hR * hk); o[9] = mt.m1 * aR.s1; o[10] = mt.m1 * aR.s2; o[11] = mt.m1 * hR;
+    o[12] = mt.m2 * (aT.s0 + hT * hk) + mt.m0; o[13] = mt.m2 * aT.s1; o[14] = mt.m2 * aT.s2; o[15] = mt.m2 * hT;
+    o[16] = mt.m1 * hL * dk; o[17] = mt.m1 * hR * dk; o[18] = mt.m2 * hT * dk;
+    o[19] = Lf * mt.m3; o[20] = Lf * mt.m4;
+    o[21] = sm * qx; o[22] = sm * qy;
+    o[23] = Lf * (prim.s0 * (lam / kPi));
     o[24] = g0.s0; o[25] = g0.s1; o[26] = g0.s2; o[27] = g0.s3;
-    o[28] = B.side_length[own * 2 + L.face]; o[29] = n.x; o[30] = n.y; o[31] = 0.0;
+    o[28] = Lf; o[29] = n.x; o[30] = n.y; o[31] = 0.0;
 }
 
 // minimum wavefronts per SIMD asked of the flux pass: the FP32 variants with a
@@ -599,14 +615,14 @@ __global__ void __launch_bounds__(64 * FW) __attribute__((amdgpu_waves_per_eu(kF
     FaceGeo g = face_geo(B, L.gi, L.gj, L.face);
     uniform_geo(g);
     const Stencil<T> stn = stencil<T, M>(C, B.f, g);
+    // the face's folded flux constants (k_faceS)
     const double *sv = fstate + fid * kFT;
-    const d4 prim = mk4(sv[0], sv[1], sv[2], sv[3]);
-    const d4 aL = mk4(sv[4], sv[5], sv[6], sv[7]), aR = mk4(sv[8], sv[9], sv[10], sv[11]);
-    const d4 aT = mk4(sv[12], sv[13], sv[14], sv[15]);
-    const double m0 = sv[16], m1 = sv[17], m2 = sv[18], m3 = sv[19], m4 = sv[20];
-    const double Mxi2 = sv[21];
-    const d2 Qf = mk2(sv[22], sv[23]);
-    const double Lf = sv[28];
+    const double U = sv[0], V = sv[1], lam = sv[2], hk = sv[3];
+    const d4 QL = mk4(sv[4], sv[5], sv[6], sv[7]), QR = mk4(sv[8], sv[9], sv[10], sv[11]);
+    const d4 QT = mk4(sv[12], sv[13], sv[14], sv[15]);
+    const double DL = sv[16], DR = sv[17], DT = sv[18], m3 = sv[19], m4 = sv[20];
+    const d2 SQ = mk2(sv[21], sv[22]);
+    const double rl = sv[23], Lf = sv[28];
     const d2 n = mk2(sv[29], sv[30]);
     // separable Maxwellian on a tensor grid with an axis-aligned normal:
     // exp(-lam |uv - (U,V)|^2) = Ei[v / nvt] * Ej[v % nvt], one table pair per wave in LDS
@@ -615,12 +631,12 @@ __global__ void __launch_bounds__(64 * FW) __attribute__((amdgpu_waves_per_eu(kF
     if (sep) {
         const bool wtype = (n.y == 0.0);
         if (lane < C.nut) {
-            const double a = wtype ? n.x * C.qu[lane] : -n.y * C.qu[lane], c = wtype ? prim.s1 : prim.s2;
-            Ei[lane] = exp(-prim.s3 * ((a - c) * (a - c)));
+            const double a = wtype ? n.x * C.qu[lane] : -n.y * C.qu[lane], c = wtype ? U : V;
+            Ei[lane] = exp(-lam * ((a - c) * (a - c)));
         }
         if (lane < C.nvt) {
-            const double a = wtype ? n.x * C.qv[lane] : n.y * C.qv[lane], c = wtype ? prim.s2 : prim.s1;
-            Ej[lane] = exp(-prim.s3 * ((a - c) * (a - c)));
+            const double a = wtype ? n.x * C.qv[lane] : n.y * C.qv[lane], c = wtype ? V : U;
+            Ej[lane] = exp(-lam * ((a - c) * (a - c)));
         }
     }
     // the tables are private to this wavefront: order its LDS writes before its reads
@@ -628,16 +644,7 @@ __global__ void __launch_bounds__(64 * FW) __attribute__((amdgpu_waves_per_eu(kF
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     AS1 V2<T> *fout = (AS1 V2<T> *)(L.face == SOUTH ? B.fluxS : B.fluxW) + g.own * (size_t)C.NV;
-    const double rl = prim.s0 * (prim.s3 / kPi);
-    const double hk = C.K / (2.0 * prim.s3);
-    const double sfac = 0.8 * (1 - C.Pr) * (prim.s3 * prim.s3) / prim.s0;
-    // per-velocity flux (distFlux, clUGKS.c:663-774) with the slope terms
-    // folded: for a = aL / aR / aT and P_a = a0 + a1 u + a2 w + a3 |u|^2 / 2,
-    //   the g part of the slope term is f0g (P_a + a3 hk / 2) and the h part
-    //   f0g (hk P_a + a3 Mxi2 / 2), with f0h = hk f0g
-    const double hL = 0.5 * aL.s3, hR = 0.5 * aR.s3, hT = 0.5 * aT.s3;
-    const double caL = hL * hk, caR = hR * hk, ct = hT * hk;
-    const double cmL = hL * Mxi2, cmR = hR * Mxi2, cmt = hT * Mxi2;
+    const double lam2 = 2.0 * lam;
     double mf[4] = {0.0, 0.0, 0.0, 0.0};
     // tensor indices of v = lane + 64 k, advanced without integer division
     int iu = 0, jv = 0, diu = 0, djv = 0;
@@ -653,41 +660,40 @@ __global__ void __launch_bounds__(64 * FW) __attribute__((amdgpu_waves_per_eu(kF
         }
         P2<T> ff, sg;
         reconstruct_from<T, M>(C, g, delta, stv, ff, sg);
-        const double ux = uv.x, uy = uv.y, cx = ux - prim.s1, cy = uy - prim.s2, c2 = cx * cx + cy * cy;
-        const double E = sep ? Ei[iu] * Ej[jv] : exp(-prim.s3 * c2);
+        const double ux = uv.x, uy = uv.y, cx = ux - U, cy = uy - V, c2 = cx * cx + cy * cy;
+        const double E = sep ? Ei[iu] * Ej[jv] : exp(-lam * c2);
         if (sep) {
             iu += diu; jv += djv;
             if (jv >= C.nvt) { jv -= C.nvt; iu++; }
         }
-        const double f0g = wv * (rl * E);
-        const double sh = sfac * (cx * Qf.x + cy * Qf.y), part = 2 * prim.s3 * c2;
-        const double uu = ux * ux + uy * uy, ux2 = ux * ux;
-        const double PL = aL.s0 + aL.s1 * ux + aL.s2 * uy + hL * uu, PR = aR.s0 + aR.s1 * ux + aR.s2 * uy + hR * uu;
-        const double Pa = delta ? PL : PR, ca = delta ? caL : caR, cma = delta ? cmL : cmR;
-        const double Pt = aT.s0 + aT.s1 * ux + aT.s2 * uy + hT * uu;
+        const double f0 = wv * (rl * E);                       // Lf f0 of g; hk f0 is h's
+        const double uu = ux * ux + uy * uy;
+        const double PL = QL.s0 + QL.s1 * ux + QL.s2 * uy + QL.s3 * uu, PR = QR.s0 + QR.s1 * ux + QR.s2 * uy + QR.s3 * uu;
+        const double Qa = delta ? PL : PR, Da = delta ? DL : DR;
+        const double Bg = fma(ux, Qa, QT.s0 + QT.s1 * ux + QT.s2 * uy + QT.s3 * uu);
+        const double S = cx * SQ.x + cy * SQ.y, SP4 = S * fma(lam2, c2, -4.0), SP2 = fma(2.0, S, SP4);
         const double fdg = (double)ff.x, fdh = (double)ff.y, slg = (double)sg.x, slh = (double)sg.y;
-        const double tg = m3 * fdg - m4 * ux * slg, th = m3 * fdh - m4 * ux * slh;
-        const double shg = sh * (part - 4), shh = sh * (part - 2);
-        const double rg = f0g * (m0 * ux * (1.0 + shg) + m1 * ux2 * (Pa + ca) + m2 * ux * (Pt + ct)) + ux * tg;
-        const double rh = f0g * (m0 * ux * hk * (1.0 + shh) + m1 * ux2 * (hk * Pa + cma) + m2 * ux * (hk * Pt + cmt)) +
-                          ux * th;
-        if constexpr (kDeferStore<T>) pend = V2<T>{(T)(Lf * rg), (T)(Lf * rh)};
-        else fout[v] = V2<T>{(T)(Lf * rg), (T)(Lf * rh)};
+        const double m4u = m4 * ux;
+        const double tg = m3 * fdg - m4u * slg, th = m3 * fdh - m4u * slh;
+        const double rg = ux * fma(f0, Bg + SP4, tg);
+        const double rh = ux * fma(f0, fma(hk, Bg + SP2, fma(ux, Da, DT)), th);
+        if constexpr (kDeferStore<T>) pend = V2<T>{(T)rg, (T)rh};
+        else fout[v] = V2<T>{(T)rg, (T)rh};
         // macroFlux (clUGKS.c:607-661): the Shakhov and the non-equilibrium parts
-        const double X = ux * (m0 * shg * f0g + tg);
+        const double X = ux * fma(SP4, f0, tg);
         mf[0] += X;
         mf[1] += ux * X;
         mf[2] += uy * X;
-        mf[3] += 0.5 * (uu * X + ux * (m0 * shh * (hk * f0g) + th));
+        mf[3] += 0.5 * (uu * X + ux * fma(SP2, hk * f0, th));
     });
     if (last >= 0) fout[last] = pend;
 #pragma unroll
     for (int q = 0; q < 4; q++) mf[q] = wave_sum(mf[q]);
     if (lane == 0) {
-        d4 o = mk4(mf[0], mf[1], mf[2], mf[3]) + mk4(sv[24], sv[25], sv[26], sv[27]);
+        d4 o = mk4(mf[0], mf[1], mf[2], mf[3]) + Lf * mk4(sv[24], sv[25], sv[26], sv[27]);
         const d2 gl = to_global(mk2(o.s1, o.s2), n);
         o.s1 = gl.x; o.s2 = gl.y;
-        (L.face == SOUTH ? B.fmS : B.fmW)[g.own] = Lf * o;
+        (L.face == SOUTH ? B.fmS : B.fmW)[g.own] = o;
     }
 }
 
