@@ -595,8 +595,9 @@ __global__ void __launch_bounds__(kT) k_faceS(const FBlock *__restrict__ blocks,
 }
 
 // minimum wavefronts per SIMD asked of the flux pass: the FP32 variants with a
-// fixed reconstruction fit five without spilling; the others are left to the compiler
-template <typename T, int M> constexpr int kFaceCWaves = (sizeof(T) == 4 && M >= 0) ? 5 : 1;
+// fixed reconstruction run six (80 VGPRs; WENO5 spills 4 registers there and
+// is still faster than at five); the others are left to the compiler
+template <typename T, int M> constexpr int kFaceCWaves = (sizeof(T) == 4 && M >= 0) ? 6 : 1;
 // the FP64 flux stores one iteration late (the FP32 variant would spill at five waves)
 template <typename T> constexpr bool kDeferStore = sizeof(T) == 8;
 template <typename T, int FW, int M>
