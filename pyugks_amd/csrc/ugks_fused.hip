@@ -92,10 +92,27 @@ __device__ __forceinline__ long long xcd_remap(long long id, long long n) {
 #endif
 }
 
+// the last block whose first item is at or before item (blocks without items
+// are passed over), by bisection: a batch of many blocks costs each wavefront
+// log2(nb) dependent loads, not nb
 __device__ __forceinline__ int find_block(const long long *off, int nb, long long item) {
-    int b = 0;
-    while (b + 1 < nb && off[b + 1] <= item) b++;
-    return b;
+    int lo = 0, hi = nb - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= item) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+// the same over a list of items ordered by their start
+template <typename Item> __device__ __forceinline__ int find_item(const Item *items, int nitems, long long item) {
+    int lo = 0, hi = nitems - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (items[mid].start <= item) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
 }
 
 template <typename T> __device__ __forceinline__ P2<T> ld(const void *p, size_t k) { return ((const P2<T> *)p)[k]; }
@@ -708,9 +725,7 @@ __global__ void __launch_bounds__(kT) k_wall(const FBlock *__restrict__ blocks, 
     __shared__ P2<T> wd[NPT * kT];   // reflected distribution (clAdsorb.c:377) for the specular mirror lookup
     __shared__ double sc[8];
     const long long item = blockIdx.x;
-    int it = 0;
-    while (it + 1 < nitems && items[it + 1].start <= item) it++;
-    const WallItem W = items[it];
+    const WallItem W = items[find_item(items, nitems, item)];
     const FBlock &B = blocks[W.block];
     const Consts &C = B.c;
     const int face = W.wall;
@@ -1203,9 +1218,7 @@ __global__ void __launch_bounds__(kT) k_ghost(const FBlock *__restrict__ blocks,
     const long long item = (long long)blockIdx.x * kWavesT + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (item >= nghosts) return;
     const int lane = threadIdx.x & 63;
-    int it = 0;
-    while (it + 1 < nitems && items[it + 1].start <= item) it++;
-    const GhostItem GI = items[it];
+    const GhostItem GI = items[find_item(items, nitems, item)];
     const FBlock &B = blocks[GI.block];
     const Consts &C = B.c;
     const int face = GI.face;
