@@ -20,7 +20,8 @@
  *   - the velocity mirrors (flip_distribution) of or_edgeExchange (its index
  *     maps for all sixteen face pairs, with the orientation flip, are pinned
  *     geometrically; the pure shifts also by the 2x2 split test),
- *   - or_edgeOutflow,
+ *   - or_edgeOutflow on north and south faces (east and west are pinned by
+ *     the outflow-pressure fixed point),
  *   - or_adsorbingWallS_P2 (specular re-emission; the diffuse and the
  *     Cercignani-Lampis ones are pinned),
  *   - the momentum and energy columns of or_wallMassEnergyFluxes (the mass

@@ -762,6 +762,48 @@ def test_constant_and_inflow_ghosts_carry_the_prescribed_state():
     assert np.allclose(V, 0.03, atol=1e-3) and np.allclose(T, 1.1, rtol=1e-3)
 
 
+def test_outflow_ghosts_hold_the_interior_state_at_the_outflow_pressure():
+    """edgeOutflow (clGhostRoutines.c:755-839) on the east and west faces:
+    density and velocity come from the adjacent interior cell and the
+    temperature from the prescribed pressure, P = rho T.  So an interior state
+    already at that pressure is carried unchanged into the ghost cells (a
+    fixed point), and a higher outflow pressure raises only T, by P / rho.
+    Checked through the ghost cells' discrete moments (41 x 41 Newton-Cotes).
+    A velocity taken with the wrong sign or frame, or T taken from the wrong
+    ratio, fails.  (On the north and south faces the reference keeps the
+    velocity in the face frame, so there the state is not carried over as it
+    is; that is not pinned.)"""
+    b = _ghost_block(40)
+    c, (quad, _w, _a, _b2, _c2) = b.consts, b.tables
+    G, NV = c["ghost"], c["NV"]
+    e = np.asarray(quad).reshape(NV, 2)
+    rho0, U0, V0, T0 = 1.2, 0.25, -0.04, 0.95
+    b["Q"][:] = 0.0
+    b["macro"][:] = [rho0, U0, V0, 1.0 / T0]
+    wp = b["wall_prop"]
+    wp[:, bcd.EAST] = [2.0, 0.0, 0.0, 2.0 / (rho0 * T0)]      # P = rho / lambda = rho0 T0
+    wp[:, bcd.WEST] = [1.0, 0.0, 0.0, 1.0 / 1.5]              # P = 1.5
+    f = b["f"]
+    _call("or_edgeOutflow", b, f, bcd.EAST, b["normal"], b["macro"], b["Q"], wp)
+    _call("or_edgeOutflow", b, f, bcd.WEST, b["normal"], b["macro"], b["Q"], wp)
+
+    def prim(x):
+        rho = np.sum(x[..., 0], -1)
+        U = np.sum(x[..., 0] * e[:, 0], -1) / rho
+        V = np.sum(x[..., 0] * e[:, 1], -1) / rho
+        E = 0.5 * np.sum((e[:, 0] ** 2 + e[:, 1] ** 2) * x[..., 0] + x[..., 1], -1)
+        T = (E / rho - 0.5 * (U ** 2 + V ** 2)) * 2.0 / (2.0 + c["K"]) / 0.5
+        return rho, U, V, T
+
+    for k in range(G):
+        rho, U, V, T = prim(f[-1 - k, G:-G])
+        assert np.allclose(rho, rho0, rtol=1e-3) and np.allclose(U, U0, atol=1e-3)
+        assert np.allclose(V, V0, atol=1e-3) and np.allclose(T, T0, rtol=1e-3)
+        rho, U, V, T = prim(f[k, G:-G])
+        assert np.allclose(rho, rho0, rtol=1e-3) and np.allclose(U, U0, atol=1e-3)
+        assert np.allclose(V, V0, atol=1e-3) and np.allclose(T, 1.5 / rho0, rtol=1e-3)
+
+
 def test_residual_and_edge_mass_flow_closed_forms():
     """getResidual is the squared change of the primary variables;
     edgeMassFlow is (n.U) rho L with the outward normal (clUGKS.c:1259-1337)."""
