@@ -24,8 +24,6 @@
  *     the outflow-pressure fixed point),
  *   - or_adsorbingWallS_P2 (specular re-emission; the diffuse and the
  *     Cercignani-Lampis ones are pinned),
- *   - the momentum and energy columns of or_wallMassEnergyFluxes (the mass
- *     columns are pinned by the zero-net-flux test),
  *   - the clamp branches of or_adsorbingWall_P1 beyond the mass balance and
  *     coverage bounds that are checked,
  *   - the walk of or_langmuir_table at degenerate edge crossings (the walk

@@ -380,6 +380,34 @@ def test_diffuse_wall_zero_net_mass_flux():
         assert np.max(np.abs(net)) < 1e-13 * scale
 
 
+def test_wall_flux_columns_of_a_gas_at_rest_are_its_half_range_fluxes():
+    """wallMassEnergyFluxes (clUGKS.c:1019-1147), every column: in a cavity at
+    rest at the walls' temperature the gas stays the resting Maxwellian, and
+    both halves at each wall are that Maxwellian, so each wall face's in and
+    out columns are L dt times its half-range fluxes: mass rho / (2 sqrt(pi
+    lam)), normal momentum rho / (4 lam), tangential momentum 0, energy
+    rho (3 + K) / (8 sqrt(pi) lam^1.5).  To quadrature accuracy (41 x 41
+    Newton-Cotes); a dropped h term, a wrong power of u or a swapped column
+    fails."""
+    o = run(cases.cavity(8, ("NC", 40), "vanLeer", 0.1, lid=0.0, wall_T=1.0), 1)
+    b = o.blocks[0]
+    wf = b["wall_fluxes"]
+    m = b["macro"]
+    rho, lam = float(np.mean(m[:, :, 0])), float(np.mean(m[:, :, 3]))
+    # (the discrete gas and the wall Maxwellian differ at quadrature level: a 3e-7 stir)
+    assert np.allclose(m[:, :, 1:3], 0.0, atol=1e-6)
+    K = b.consts["K"]
+    Ldt = (1.0 / 8) * o.dt
+    mass = Ldt * rho / (2 * np.sqrt(np.pi * lam))
+    mom = Ldt * rho / (4 * lam)
+    nrg = Ldt * rho * (3 + K) / (8 * np.sqrt(np.pi) * lam ** 1.5)
+    for face in range(4):
+        col = wf[:8, face, :]
+        for k, want in ((0, mass), (1, mass), (2, mom), (3, mom), (6, nrg), (7, nrg)):
+            assert np.allclose(np.abs(col[:, k]), want, rtol=2e-3), (face, k)
+        assert np.allclose(col[:, 4:6], 0.0, atol=1e-5 * mass), face
+
+
 def test_adsorbing_without_sticking_equals_diffuse_wall():
     gd = cases.cavity(8, ("NC", 12), "WENO5", 0.05, lid=0.1, wall_T=1.0)
     ga = copy.deepcopy(gd)
