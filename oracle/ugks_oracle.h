@@ -22,8 +22,8 @@
  *     geometrically; the pure shifts also by the 2x2 split test),
  *   - or_edgeOutflow on north and south faces (east and west are pinned by
  *     the outflow-pressure fixed point),
- *   - or_adsorbingWallS_P2 (specular re-emission; the diffuse and the
- *     Cercignani-Lampis ones are pinned),
+ *   - or_adsorbingWallS_P2 with sticking (without it, the elastic specular
+ *     reflection is pinned; the diffuse and Cercignani-Lampis ones fully),
  *   - the clamp branches of or_adsorbingWall_P1 beyond the mass balance and
  *     coverage bounds that are checked,
  *   - the walk of or_langmuir_table at degenerate edge crossings (the walk

@@ -417,6 +417,24 @@ def test_adsorbing_without_sticking_equals_diffuse_wall():
     assert d < 1e-13
 
 
+def test_specular_wall_without_sticking_reflects_elastically():
+    """adsorbingWallS_P2 (clAdsorb.c:656-816) with no sticking (k_f = 0) is
+    specular reflection: every outgoing velocity carries what came in at its
+    mirror image.  On the mirror-symmetric velocity grid, mass, tangential
+    momentum and energy then leave the wall as they arrived, to rounding,
+    whatever the wall temperature (1.3 here, against a gas at 1): a diffuse
+    re-emission at the wall temperature, or a mirror on the wrong axis,
+    fails on the energy or the tangential momentum."""
+    g = cases.cavity(8, ("NC", 12), "WENO5", 0.05, lid=0.3, wall_T=1.0)
+    g.blocks[0].bc_list[bcd.SOUTH] = bcd.AdsorbingBC(D=1.0, T=1.3, k_f=0.0, adsorb=0.5, reflect_type="S")
+    o = run(g, 3)
+    wf = o.blocks[0]["wall_fluxes"][:8, bcd.SOUTH, :]
+    scale = np.max(np.abs(wf[:, 0]))
+    for k_in, k_out in ((0, 1), (4, 5), (6, 7)):
+        assert np.max(np.abs(np.abs(wf[:, k_in]) - np.abs(wf[:, k_out]))) < 1e-13 * scale, k_in
+    assert np.max(np.abs(wf[:, 4])) > 1e-6 * scale        # the gas does slide along the wall
+
+
 def test_adsorbing_wall_mass_balance_and_coverage_bounds():
     g = cases._base("WENO5", 0.05)
     cases.newton_cotes(g, 24)
