@@ -235,6 +235,7 @@ struct FaceGeo {
     double dist[2];     // |midside - c[MID]|
     double rdist[2];    // 1 / dist
     double rd1[2];      // 1 / |c[MID] - c[MID-1]| (vanLeer divisor)
+    float fdist[2], frdist[2], frd1[2];   // the same in float, for the FP32 reconstruction
 };
 
 __device__ inline FaceGeo face_geo(const FBlock &B, int gi, int gj, int face) {
@@ -251,6 +252,9 @@ __device__ inline FaceGeo face_geo(const FBlock &B, int gi, int gj, int face) {
         g.dist[delta] = length(ms - c0);
         g.rdist[delta] = 1.0 / g.dist[delta];
         g.rd1[delta] = 1.0 / length(c0 - c1);
+        g.fdist[delta] = (float)g.dist[delta];
+        g.frdist[delta] = (float)g.rdist[delta];
+        g.frd1[delta] = (float)g.rd1[delta];
     }
     return g;
 }
@@ -305,7 +309,20 @@ __device__ __forceinline__ void uniform_geo(FaceGeo &g) {
         g.rd1[k] = uniform_d(g.rd1[k]);
         __asm__ volatile("" : "+v"(g.rdist[k]));
         __asm__ volatile("" : "+v"(g.rd1[k]));
+        // float copies held in VGPRs: the per-lane pick of one of two is then a
+        // single select, not two moves, a select and a conversion per velocity
+        g.fdist[k] = (float)g.dist[k];
+        g.frdist[k] = (float)g.rdist[k];
+        g.frd1[k] = (float)g.rd1[k];
+        __asm__ volatile("" : "+v"(g.fdist[k]));
+        __asm__ volatile("" : "+v"(g.frdist[k]));
+        __asm__ volatile("" : "+v"(g.frd1[k]));
     }
+}
+// the per-face distance of upwind choice delta in the reconstruction's precision
+template <typename T> __device__ __forceinline__ T geo_pick(const double (&d)[2], const float (&f)[2], int delta) {
+    if constexpr (sizeof(T) == 4) return delta ? f[1] : f[0];
+    else return delta ? d[1] : d[0];
 }
 template <typename T> __device__ __forceinline__ P2<T> sld(const Stencil<T> &st, unsigned rel, unsigned vb) {
     const V2<T> x = *(const AS1 V2<T> *)(st.base + (rel * st.sb + vb));
@@ -339,20 +356,20 @@ template <typename T, int M = -1>
 __device__ __forceinline__ void reconstruct_from(const Consts &C, const FaceGeo &g, int delta, const StenVals<T> &x,
                                                  P2<T> &ff, P2<T> &sg, P2<T> *fl = nullptr, P2<T> *fr = nullptr) {
     if (method_of<M>(C) == 1) {
-        const T dist = (T)(delta ? g.dist[1] : g.dist[0]), rdx = (T)(delta ? g.rdist[1] : g.rdist[0]);
+        const T dist = geo_pick<T>(g.dist, g.fdist, delta), rdx = geo_pick<T>(g.rdist, g.frdist, delta);
         weno5_pair(x.s0, x.s1, x.s2, x.s3, x.s4, rdx, sg.x, sg.y);
         ff.x = x.s2.x + sg.x * dist;
         ff.y = x.s2.y + sg.y * dist;
         if (fl) *fl = P2<T>{delta ? x.s2.x : x.s3.x, delta ? x.s2.y : x.s3.y};
         if (fr) *fr = P2<T>{delta ? x.s3.x : x.s2.x, delta ? x.s3.y : x.s2.y};
     } else {
-        const T dist = (T)(delta ? g.dist[1] : g.dist[0]);
+        const T dist = geo_pick<T>(g.dist, g.fdist, delta);
         if (method_of<M>(C) == 0) {
-            const T r1 = (T)(delta ? g.rd1[1] : g.rd1[0]);
+            const T r1 = geo_pick<T>(g.rd1, g.frd1, delta);
             sg.x = vanleer_T<T>((x.s1.x - x.s0.x) * r1, (x.s2.x - x.s1.x) * r1);
             sg.y = vanleer_T<T>((x.s1.y - x.s0.y) * r1, (x.s2.y - x.s1.y) * r1);
         } else {
-            const T r = (T)(delta ? g.rdist[1] : g.rdist[0]);
+            const T r = geo_pick<T>(g.rdist, g.frdist, delta);
             sg.x = (x.s1.x + T(0.5) * minmod_T<T>(x.s2.x - x.s1.x, x.s1.x - x.s0.x) - x.s1.x) * r;
             sg.y = (x.s1.y + T(0.5) * minmod_T<T>(x.s2.y - x.s1.y, x.s1.y - x.s0.y) - x.s1.y) * r;
         }
@@ -440,21 +457,24 @@ __device__ __forceinline__ void face_velocity_loop(const Consts &C, const Stenci
         const d2 uv = d2{ec.x * n.x + ec.y * n.y, ec.x * (-n.y) + ec.y * n.x};     // face_velocity
         const int delta = upwind(uv.x);
         StenVals<T> cur;
+        // the next velocity's table entries, or this one's again on the last
+        // round: the loads then need no branch (and no register copies around
+        // one).  The stencil loads into LDS stay conditional, so none is still
+        // in flight when the wavefront leaves the loop.
+        const int vn = v + 64 < C.NV ? v + 64 : v;
         if constexpr (LDS) {
             stencil_read<T, M>(C, lb, lane, delta, cur);
             // the reads have returned before the next loads overwrite the buffer
             __builtin_amdgcn_s_waitcnt(0xC07F);   // lgkmcnt(0)
+            e = quad_at(C, vn);
+            w = weight_at(C, vn);
             if (v + 64 < C.NV) {
-                e = quad_at(C, v + 64);
-                w = weight_at(C, v + 64);
                 vb += 64u * (unsigned)sizeof(P2<T>);
                 stencil_dma<T, M>(C, st, vb, lb);
             }
         } else {
-            if (v + 64 < C.NV) {
-                e = quad_at(C, v + 64);
-                w = weight_at(C, v + 64);
-            }
+            e = quad_at(C, vn);
+            w = weight_at(C, vn);
             stencil_load<T, M>(C, st, v, delta, cur);
         }
         body(v, ec, wc, uv, delta, cur);
@@ -1086,9 +1106,10 @@ __global__ void __launch_bounds__(kT) k_update(const FBlock *__restrict__ blocks
         const V2<T> fo = fc[v], sS = fS[v], nS_ = fS[v + nS], wW = fW[v], eW = fW[v + nE];
         const d2 e = enext;
         const double w = wnext;
-        if (v + 64 < C.NV) {
-            enext = quad_at(C, v + 64);
-            wnext = weight_at(C, v + 64);
+        {   // the next velocity's table entries (this one's again on the last round: no branch)
+            const int vn = v + 64 < C.NV ? v + 64 : v;
+            enext = quad_at(C, vn);
+            wnext = weight_at(C, vn);
         }
         const double ox = e.x - pold.s1, oy = e.y - pold.s2, nx = e.x - p.s1, ny = e.y - p.s2;
         const double co = ox * ox + oy * oy, cn = nx * nx + ny * ny;
