@@ -27,6 +27,10 @@ def small_cases():
     out["cavity_weno_nc_2x2"] = cases.cavity(16, ("NC", 12), "WENO5", 0.01, nb=(2, 2), n_cells_j=12)
     out["cavity_minmod_nc"] = cases.cavity(10, ("NC", 8), "minmod", 1.0, lid=0.3, wall_T=1.1)
     out["cavity_weno_rho_perturbed"] = cases.cavity(12, ("NC", 12), "WENO5", 0.001, rho_perturb=0.01)
+    # 33 x 33 velocities: several 256-velocity chunks per wall face, and a 17 x 64 + 1 velocity loop
+    out["cavity_weno_nv1089"] = cases.cavity(6, ("NC", 32), "WENO5", 0.01, lid=0.2, n_cells_j=5)
+    # a block two cells high between two walls
+    out["thin_two_rows"] = cases.cavity(9, ("NC", 8), "vanLeer", 0.05, lid=0.2, n_cells_j=2)
     out["channel_adsorbing"] = cases.cfg5_channel(widths=[6, 10, 8], height=8, nc=8, Kn=0.05)
     out["channel_adsorbing_cl"] = cases.cfg5_channel(widths=[7, 9], height=6, nc=8, Kn=0.05, reflect_type="CL",
                                                      alpha_n=0.7, alpha_t=0.45)
