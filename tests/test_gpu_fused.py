@@ -64,7 +64,8 @@ def test_fused_fp64_matches_oracle(hip, name):
 
 @pytest.mark.parametrize("name", ["cavity_vanleer_gh", "cavity_weno_nc_2x2", "cavity_weno_rho_perturbed",
                                   "channel_adsorbing", "channel_adsorbing_cl", "channel_adsorbing_table",
-                                  "curvilinear_weno_2x2", "join_N_W", "join_E_E"])
+                                  "curvilinear_weno_2x2", "join_N_W", "join_E_E", "cavity_weno_nv1089",
+                                  "thin_two_rows", "bounce_back"])
 def test_fused_mixed_error_vs_oracle(hip, name):
     gpu, plan, ora = build(name, "mixed")
     nsteps = 20
