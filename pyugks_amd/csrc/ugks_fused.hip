@@ -288,10 +288,6 @@ __device__ __forceinline__ d2 quad_at(const Consts &C, int v) {
     return d2{x.x, x.y};
 }
 __device__ __forceinline__ double weight_at(const Consts &C, int v) { return ((const AS1 double *)C.weight)[v]; }
-__device__ __forceinline__ d2 face_velocity_g(const Consts &C, int v, d2 n) {
-    const d2 e = quad_at(C, v);
-    return d2{e.x * n.x + e.y * n.y, e.x * (-n.y) + e.y * n.x};
-}
 // a wavefront-uniform double held in scalar registers (keeps per-face
 // constants out of VGPRs, so they are neither spilled nor recomputed in loops)
 __device__ __forceinline__ double uniform_d(double x) {
