@@ -280,6 +280,12 @@ __device__ __forceinline__ Stencil<T> stencil(const Consts &C, const void *F, co
     return st;
 }
 template <typename T> using V2 = T __attribute__((ext_vector_type(2)));
+// a store that is not read again soon (per-velocity fluxes, the new f): it
+// streams past the caches instead of evicting stencil data still in use
+template <typename T> __device__ __forceinline__ void stream_store(V2<T> *p, V2<T> x) { __builtin_nontemporal_store(x, p); }
+template <typename T> __device__ __forceinline__ void stream_store(__attribute__((address_space(1))) V2<T> *p, V2<T> x) {
+    __builtin_nontemporal_store(x, p);
+}
 
 // velocity table reads through global (not generic) pointers: they count only
 // against the vector-memory counter, so waiting for them does not also wait on LDS
@@ -689,7 +695,7 @@ __global__ void __launch_bounds__(64 * FW) __attribute__((amdgpu_waves_per_eu(kF
     int last = -1;
     face_velocity_loop<T, M, kStenViaLds<T>>(C, stn, n, lane, (AS3 char *)sbuf[wv], [&](int v, d2, double wv, d2 uv, int delta, const StenVals<T> &stv) __attribute__((always_inline)) {
         if constexpr (kDeferStore<T>) {
-            if (v >= 64) fout[v - 64] = pend;
+            if (v >= 64) stream_store<T>(fout + (v - 64), pend);
             last = v;
         }
         P2<T> ff, sg;
@@ -712,7 +718,7 @@ __global__ void __launch_bounds__(64 * FW) __attribute__((amdgpu_waves_per_eu(kF
         const double rg = ux * fma(f0, Bg + SP4, tg);
         const double rh = ux * fma(f0, fma(hk, Bg + SP2, fma(ux, Da, DT)), th);
         if constexpr (kDeferStore<T>) pend = V2<T>{(T)rg, (T)rh};
-        else fout[v] = V2<T>{(T)rg, (T)rh};
+        else stream_store<T>(fout + v, V2<T>{(T)rg, (T)rh});
         // macroFlux (clUGKS.c:607-661): the Shakhov and the non-equilibrium parts
         const double X = ux * fma(SP4, f0, tg);
         mf[0] += X;
@@ -720,7 +726,7 @@ __global__ void __launch_bounds__(64 * FW) __attribute__((amdgpu_waves_per_eu(kF
         mf[2] += uy * X;
         mf[3] += 0.5 * (uu * X + ux * fma(SP2, hk * f0, th));
     });
-    if (last >= 0) fout[last] = pend;
+    if (last >= 0) stream_store<T>(fout + last, pend);
 #pragma unroll
     for (int q = 0; q < 4; q++) mf[q] = wave_sum(mf[q]);
     if (lane == 0) {
@@ -1132,7 +1138,7 @@ __global__ void __launch_bounds__(kT) k_update(const FBlock *__restrict__ blocks
         const double nh = (fh + flh + (htn * feqNh + hto * (feqOh - fh))) * rden;
         const T tg = (T)ng, th = (T)nh;
         if (isnan(tg) || isnan(th)) nan_seen = true;
-        fc[v] = V2<T>{tg, th};
+        stream_store<T>(fc + v, V2<T>{tg, th});
         // the new cell moments (the getAL / getAR inputs of the next step)
         add_moments<T>(e, tg, th, mom);
     }
