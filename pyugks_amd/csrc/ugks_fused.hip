@@ -517,6 +517,50 @@ __device__ __forceinline__ FaceLoc face_loc(const FBlock &B, long long loc) {
     f.gj = f.mj + C.G;
     return f;
 }
+// the face_loc index of a face
+__device__ __forceinline__ long long face_index(const FBlock &B, const FaceLoc &f) {
+    return f.face == SOUTH ? (long long)f.mi * B.nSj + (f.mj - B.ob_S)
+                           : B.nS + (long long)f.mj * B.nWi + (f.mi - B.ob_W);
+}
+
+// The moment pass walks a block's faces in tiles of kTileI x kTileJ cells
+// (tiles j-fastest): each tile's S faces (j fastest), then its W faces (i
+// fastest).  The S faces form an ni x nSj rectangle (mi, mj - ob_S), the W faces
+// an nWi x nj one (mi - ob_W, mj); the same tile grid cuts both, so the two
+// kinds of face that read a cell's stencil data run at about the same time
+// and share it in the caches.  (The flux pass measured faster in the plain
+// order, and it and the state pass keep it; the moment pass writes each
+// face's sums at the face's plain index.)  loc < 2^31 (checked on the host).
+constexpr int kTileI = 4, kTileJ = 16;
+__device__ __forceinline__ FaceLoc face_loc_tiled(const FBlock &B, long long loc64) {
+    const Consts &C = B.c;
+    const int loc = (int)loc64;
+    const int ni = C.ni, nj = C.nj, nSj = B.nSj, nWi = B.nWi;
+    const int nti = (max(ni, nWi) + kTileI - 1) / kTileI, ntj = (max(nSj, nj) + kTileJ - 1) / kTileJ;
+    auto rows_before = [&](int ti) { return min(ti * kTileI, ni) * nSj + min(ti * kTileI, nWi) * nj; };
+    int ti = min(loc / (kTileI * (nSj + nj)), nti - 1);
+    while (ti > 0 && rows_before(ti) > loc) ti--;
+    while (ti + 1 < nti && rows_before(ti + 1) <= loc) ti++;
+    const int rS = max(0, min(kTileI, ni - ti * kTileI)), rW = max(0, min(kTileI, nWi - ti * kTileI));
+    const int local = loc - rows_before(ti);
+    auto cols_before = [&](int tj) { return rS * min(tj * kTileJ, nSj) + rW * min(tj * kTileJ, nj); };
+    const int per = kTileJ * (rS + rW);
+    int tj = per > 0 ? min(local / per, ntj - 1) : 0;
+    while (tj > 0 && cols_before(tj) > local) tj--;
+    while (tj + 1 < ntj && cols_before(tj + 1) <= local) tj++;
+    const int cS = max(0, min(kTileJ, nSj - tj * kTileJ));
+    const int l2 = local - cols_before(tj);
+    FaceLoc f;
+    if (l2 < rS * cS) {
+        f.face = SOUTH; f.mi = ti * kTileI + l2 / cS; f.mj = tj * kTileJ + l2 % cS + B.ob_S;
+    } else {
+        const int r = l2 - rS * cS;
+        f.face = WEST; f.mi = ti * kTileI + r % rW + B.ob_W; f.mj = tj * kTileJ + r / rW;
+    }
+    f.gi = f.mi + C.G;
+    f.gj = f.mj + C.G;
+    return f;
+}
 
 template <typename T, int FW, int M>
 __global__ void __launch_bounds__(64 * FW) k_faceA(const FBlock *__restrict__ blocks, const long long *__restrict__ off,
@@ -527,7 +571,8 @@ __global__ void __launch_bounds__(64 * FW) k_faceA(const FBlock *__restrict__ bl
     const int b = find_block(off, nb, fid);
     const FBlock &B = blocks[b];
     const Consts &C = B.c;
-    const FaceLoc L = face_loc(B, fid - off[b]);
+    const FaceLoc L = face_loc_tiled(B, fid - off[b]);
+    const long long sid = off[b] + face_index(B, L);          // where the state pass looks for this face
     FaceGeo g = face_geo(B, L.gi, L.gj, L.face);
     uniform_geo(g);
     const Stencil<T> stn = stencil<T, M>(C, B.f, g);
@@ -557,7 +602,7 @@ __global__ void __launch_bounds__(64 * FW) k_faceA(const FBlock *__restrict__ bl
 #pragma unroll
     for (int q = 0; q < kNS; q++) a[q] = wave_sum(a[q]);
     if (lane == 0) {
-        double *o = fsum + fid * kFS;
+        double *o = fsum + sid * kFS;
 #pragma unroll
         for (int q = 0; q < kNS; q++) o[q] = a[q];
         if (wantQ) B.Q[(size_t)L.mi * C.nj + L.mj] = mk2(a[13], a[14]);
@@ -1646,6 +1691,9 @@ int ugks_step_plan_create_batch(ugks_block *const *blocks, const ugks_block_buff
         F.nS = (int)((long long)C.ni * F.nSj);
         F.nWi = C.ni + 1 - F.ob_W - ot_W;
         F.wl = C.ni > C.nj ? C.ni : C.nj;
+        if ((long long)F.nS + (long long)F.nWi * C.nj >= (1LL << 31) - (long long)kTileI * kTileJ * 2) {
+            delete p; return ugks_rt::fail(-2, "step plan: too many faces in one block");
+        }
         face_off[b + 1] = face_off[b] + F.nS + (long long)F.nWi * C.nj;
         cell_off[b + 1] = cell_off[b] + (long long)C.ni * C.nj;
         all_off[b + 1] = all_off[b] + (long long)C.NI * C.NJ;
