@@ -11,7 +11,8 @@
 //             S faces).
 //   k_faceS   one thread per interior face: primary variables, slopes and
 //             time slope (getPLR/initMacroFlux), face heat flux (calcFaceQ)
-//             and the equilibrium part of the macro flux.
+//             and the equilibrium part of the macro flux, folded into the
+//             per-face constants of the flux pass.
 //   k_faceC   one wavefront per interior face: reconstruction again, macro
 //             flux (macroFlux) and the per-velocity flux (distFlux).
 //   k_wall    one workgroup per diffuse/adsorbing wall face: reconstruction,
@@ -25,12 +26,13 @@
 //   k_ghost   one wavefront per ghost cell: every BC of every block
 //             (updateBC, ugks_block.py:621-665).
 //
-// Items of all blocks are laid out back to back; a workgroup finds its block
-// from a prefix table.  f is stored as T2 = (g, h) pairs of T (double, or
-// float for the mixed-precision step); moment sums, the macroscopic update,
-// heat fluxes, equilibrium moments and the time step are FP64 in both.
-// Thread t of a workgroup owns velocities v = t + 256 k: every wavefront
-// load or store touches 64 consecutive velocities of one cell.
+// Items of all blocks are laid out back to back; a wavefront finds its block
+// by bisection of a prefix table.  f is stored as T2 = (g, h) pairs of T
+// (double, or float for the mixed-precision step); moment sums, the
+// macroscopic update, heat fluxes, equilibrium moments and the time step are
+// FP64 in both.  Lane l of a wavefront owns velocities v = l + 64 k (thread t
+// owns t + 256 k in the workgroup-per-face wall pass): every wavefront load or
+// store touches 64 consecutive velocities of one cell.
 #include <stdio.h>
 #include <string.h>
 
@@ -682,7 +684,7 @@ __global__ void __launch_bounds__(kT) k_faceS(const FBlock *__restrict__ blocks,
 // fixed reconstruction run six (80 VGPRs; WENO5 spills 4 registers there and
 // is still faster than at five); the others are left to the compiler
 template <typename T, int M> constexpr int kFaceCWaves = (sizeof(T) == 4 && M >= 0) ? 6 : 1;
-// the FP64 flux stores one iteration late (the FP32 variant would spill at five waves)
+// the FP64 flux stores one iteration late (the FP32 variant would spill more at six waves)
 template <typename T> constexpr bool kDeferStore = sizeof(T) == 8;
 template <typename T, int FW, int M>
 __global__ void __launch_bounds__(64 * FW) __attribute__((amdgpu_waves_per_eu(kFaceCWaves<T, M>))) k_faceC(const FBlock *__restrict__ blocks, const long long *__restrict__ off,
