@@ -162,10 +162,10 @@ def run_precision(gs, precision, a, upd_per_step, workload):
     achieved = b_alg * upd_per_step / (per_launch[dom] * 1e-3) / 1e9
     tag = f"{workload}_{precision}"
     roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": read_profile_table("traffic.txt", dom, tag),
+            "frac": achieved / HBM_PEAK_GBS, "traffic": read_profile_table(f"hbm_traffic_per_kernel_{workload}.txt", dom, tag),
             "kernel": dom, "kernel_ms": per_launch[dom], "b_alg_per_update": b_alg,
             "step_achieved_gbs": value * b_alg / 1e9}
-    valu = read_profile_table("valu.txt", dom, tag)
+    valu = read_profile_table(f"valu_per_kernel_{workload}.txt", dom, tag)
     if valu is not None:
         roof["valu_insts"] = valu
         roof["valu_frac"] = valu / (per_launch[dom] * 1e-3 * CUS * CLOCK_HZ)
