@@ -15,7 +15,7 @@ import pytest
 from helpers import BUFFERS, SCALE_FLOOR, rel_err
 from pyugks_amd import bc as bcd
 from pyugks_amd import cases, joins
-from pyugks_amd.data import FlowCondition, UGKSData, cartesian_superblock
+from pyugks_amd.data import FlowCondition, cartesian_superblock
 
 TOL = 1e-12
 pytestmark = pytest.mark.gpu

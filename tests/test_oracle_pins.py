@@ -37,7 +37,7 @@ from scipy import integrate, special
 import oracle as O
 from pyugks_amd import bc as bcd
 from pyugks_amd import cases
-from pyugks_amd.data import (BlockDef, FlowCondition, UGKSData, bake_e15, cartesian_superblock, newton_coeff,
+from pyugks_amd.data import (BlockDef, FlowCondition, bake_e15, cartesian_superblock, newton_coeff,
                              py2_str)
 
 H = O.helpers()
