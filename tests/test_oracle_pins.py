@@ -228,6 +228,16 @@ def test_limiters():
     r = H.or_WENO5(S, dx)
     assert abs(r.x - (6.25 - S[2].x) / dx) < 1e-13
     assert abs(r.y - ((3.0 - 5.0 + 0.5 * 6.25) - S[2].y) / dx) < 1e-13
+    # WENO5 on cell averages of sin: fifth order away from critical points.
+    # Exactness on quadratics holds for any linear weights; the order pins them
+    # (reversed weights (0.3, 0.6, 0.1) give third order, a ratio near 8).
+    x0, errs = 0.3, []
+    for h in (0.16, 0.08, 0.04, 0.02):
+        avg = [(math.cos(x0 + (k - 2.5) * h) - math.cos(x0 + (k - 1.5) * h)) / h for k in range(5)]
+        r = H.or_WENO5((O.V2 * 5)(*[O.V2(a, -a) for a in avg]), 1.0)
+        errs.append(abs(avg[2] + r.x - math.sin(x0 + 0.5 * h)))
+        assert r.y == -r.x
+    assert all(25.0 < errs[i] / errs[i + 1] < 40.0 for i in range(3)), errs
     # NND: minmod of the one-sided differences, halved
     S3 = (O.V2 * 3)(O.V2(1.0, 1.0), O.V2(2.0, 2.0), O.V2(2.5, 1.0))
     r = H.or_NND(S3, 0.5)
