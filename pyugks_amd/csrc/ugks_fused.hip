@@ -755,7 +755,14 @@ __global__ void __launch_bounds__(64 * FW) __attribute__((amdgpu_waves_per_eu(kF
         }
         const double f0 = wv * (rl * E);                       // Lf f0 of g; hk f0 is h's
         const double uu = ux * ux + uy * uy;
-        const double PL = QL.s0 + QL.s1 * ux + QL.s2 * uy + QL.s3 * uu, PR = QR.s0 + QR.s1 * ux + QR.s2 * uy + QR.s3 * uu;
+        double PL = QL.s0 + QL.s1 * ux + QL.s2 * uy + QL.s3 * uu, PR = QR.s0 + QR.s1 * ux + QR.s2 * uy + QR.s3 * uu;
+        // FP32 storage: both sides' polynomials are evaluated and one is picked
+        // per lane.  Left to itself the compiler picks the coefficients instead,
+        // which costs two moves per select (a select takes one scalar operand):
+        // 19 more VALU per velocity.  The FP64 variant keeps the compiler's
+        // form: with fewer VGPRs it runs four waves per SIMD, and measured
+        // slower there (1.43 -> 1.50 ms on 2x2 blocks of 128^2 cells, 33^2 velocities).
+        if constexpr (sizeof(T) == 4) __asm__ volatile("" : "+v"(PL), "+v"(PR));
         const double Qa = delta ? PL : PR, Da = delta ? DL : DR;
         const double Bg = fma(ux, Qa, QT.s0 + QT.s1 * ux + QT.s2 * uy + QT.s3 * uu);
         const double S = cx * SQ.x + cy * SQ.y, SP4 = S * fma(lam2, c2, -4.0), SP2 = fma(2.0, S, SP4);
