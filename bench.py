@@ -209,9 +209,18 @@ def cpu_baseline(gs, a, upd_per_step):
         o.run(1)
     one = time.perf_counter() - t0
     n = max(1, int(a.cpu_seconds / max(one, 1e-3)) - 1)
+    # long samples run in chunks of about 30 s, each reported as it ends
+    chunk = max(1, int(30.0 / max(one, 1e-3)))
     t0 = time.perf_counter()
-    for o in oras:
-        o.run(n)
+    done = 0
+    while done < n:
+        k = min(chunk, n - done)
+        for o in oras:
+            o.run(k)
+        done += k
+        if n > chunk:
+            print(f"[bench] cpu baseline: {done}/{n} steps, {time.perf_counter() - t0:.0f} s",
+                  file=sys.stderr, flush=True)
     dt = time.perf_counter() - t0
     out = {"value": upd_per_step * n / dt, "unit": "cell-velocity updates/s", "cores": cores, "kind": "port",
            "sample": f"{n} steps of the same workload after 1 untimed step, FP64 C/OpenMP oracle, {dt:.1f} s "
