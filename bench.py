@@ -1,26 +1,30 @@
 #!/usr/bin/env python3
 """Benchmark of the UGKS time step on one MI355X.
 
-Default workload: the lid-driven cavity split into 2x2 blocks of 128x128
-cells (256x256 in total) with ghost exchange between the blocks, 33x33
-Newton-Cotes velocities on [-6,6]^2 (the "32x32" grid after the reference's
-n -> floor(n/4)*4+1 rounding), WENO5, Kn = 0.01, lid 0.15, CFL 0.8, FP64.
-Synthetic input: the initial state is the Maxwellian at rho = 1, U = 0,
-T = 1 (no data files).  Other workloads (--workload): cavity_64 (64^2,
+Default workload (batch64): 64 independent lid-driven cavities of uneven
+size in ONE plan, drawn from a fixed seed: cells per side from {32, 64, 96,
+128, 192, 256}, Newton-Cotes velocity grids "16/24/32/40" per axis (17^2,
+25^2, 33^2, 41^2 after the reference's n -> floor(n/4)*4+1 rounding) on
+[-6,6]^2, Kn log-uniform in [1e-3, 1], lid speed in [0.05, 0.3], wall T in
+[0.8, 1.2], WENO5, CFL 0.8.  It is the mixed FP32/FP64 batched step the
+metric is quoted on; the same batch also runs in plain FP64 beside it.
+Synthetic input: each case starts from the Maxwellian at rho = 1, U = 0,
+T = 1 (no data files).  Other workloads (--workload): cavity_2x2 (2x2 blocks
+of 128^2 cells, 33^2 velocities, WENO5, Kn 0.01), cavity_64 (64^2,
 Gauss-Hermite 24^2, vanLeer, Kn 0.075), cavity_512 (512^2, Newton-Cotes
-49^2, rho perturbed by 1%, --kn), batch64 (64 seeded cavities of uneven
-size in one plan), channel (adsorbing-wall channel, 8 blocks, 1024x256,
-41^2 velocities).
+49^2, rho perturbed by 1%, --kn), channel (adsorbing-wall channel, 8
+blocks, 1024x256, 41^2 velocities).
 
 A step is one full time step of every block of every case (flux, update,
 time step, ghost exchange) through the fused path, replayed as a HIP graph
 (--no-graph: eager launches); the metric is cell-velocity updates per
 second = sum(ni*nj*Nv) * steps / seconds, with all state resident in HBM.
-The same workload is also run with the mixed FP32/FP64 step and its error
-against the FP64 step is reported (L-inf and L2 of rho, U, T, q and the
-mass drift of both), and the FP64 CPU oracle is timed on a bounded sample
-on the host cores, with the FP64 GPU state compared to it after the same
-steps.
+`value` is the mixed-precision rate; the FP64 rate, the mixed step's error
+against the FP64 step (L-inf and L2 of rho, U, T, q) and the mass drift of
+both are reported beside it.  The FP64 CPU oracle is timed on a bounded
+sample of the same workload (all host threads of this process's CPU share,
+and one thread; median of three runs each), with the FP64 GPU state
+compared to it after the same steps.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
 """
@@ -46,14 +50,14 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", choices=WORKLOADS, default="cavity_2x2")
+    ap.add_argument("--workload", choices=WORKLOADS, default="batch64")
     ap.add_argument("--cells", type=int, default=128, help="cavity_2x2: cells per block side")
     ap.add_argument("--nc", type=int, default=32, help="cavity_2x2: Newton-Cotes points per axis before rounding")
     ap.add_argument("--kn", type=float, default=0.075, help="cavity_512: Knudsen number")
     ap.add_argument("--precision", choices=("both", "fp64", "mixed"), default="both")
     ap.add_argument("--no-graph", action="store_true", help="time eager launches instead of graph replays")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=5.0, help="target length of one CPU sample run")
     ap.add_argument("--profile-steps", type=int, default=5, help="steps timed per kernel with HIP events")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run the plain copy kernel on MIB MiB (byte-counter calibration) and exit")
@@ -80,11 +84,14 @@ def calibrate_copy(mib, reps=5):
 
 
 def host_cores():
+    """Threads for the all-core CPU sample: the CPUs this process may run on,
+    held to OMP_NUM_THREADS when that is set (the host's CPU share for one
+    GPU); also the machine's nproc."""
     n = len(os.sched_getaffinity(0))
     env = os.environ.get("OMP_NUM_THREADS")
-    if env and env.isdigit():
+    if env and env.isdigit() and int(env) > 0:
         n = min(n, int(env))
-    return max(1, min(n, 16))
+    return max(1, n), os.cpu_count() or 1
 
 
 def cpu_model():
@@ -98,21 +105,28 @@ def cpu_model():
     return "unknown"
 
 
-def read_profile_table(fname, kernel, workload):
-    """A per-launch figure for (kernel, workload) from a summary committed
-    under profiles/ (lines "kernel workload value ...")."""
+def read_profile_table(fname, workload):
+    """Per-launch figures {kernel: value} for one workload from a counter
+    summary committed under profiles/ (lines "kernel workload value ..."), and
+    the source commit the summary says it was measured on ("# source commit
+    <hash>").  These are NOT measured by this run: they are labelled with
+    their file and commit wherever they are reported."""
     path = os.path.join(ROOT, "profiles", fname)
     if not os.path.exists(path):
-        return None
+        return {}, None
+    out, commit = {}, None
     with open(path) as fh:
         for line in fh:
+            if line.startswith("# source commit"):
+                commit = line.split()[-1]
+                continue
             parts = line.split()
-            if len(parts) >= 3 and parts[0] == kernel and parts[1] == workload:
+            if len(parts) >= 3 and parts[1] == workload and not line.startswith("#"):
                 try:
-                    return float(parts[2])
+                    out[parts[0]] = float(parts[2])
                 except ValueError:
-                    return None
-    return None
+                    pass
+    return out, commit
 
 
 def build_cases(a):
@@ -160,16 +174,40 @@ def run_precision(gs, precision, a, upd_per_step, workload):
     per_launch = {k: v / a.profile_steps for k, v in kern.items()}
     dom = max(per_launch, key=per_launch.get)
     b_alg = 32.0 if precision == "fp64" else 16.0
-    achieved = b_alg * upd_per_step / (per_launch[dom] * 1e-3) / 1e9
+    step_s = wall_s / a.steps
+    # whole step: the compulsory bytes of one step (1 read + 1 write of f per
+    # update) over the measured time of the step (all eight launches)
+    achieved = b_alg * upd_per_step / step_s / 1e9
     tag = f"{workload}_{precision}"
     roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": read_profile_table(f"hbm_traffic_per_kernel_{workload}.txt", dom, tag),
-            "kernel": dom, "kernel_ms": per_launch[dom], "b_alg_per_update": b_alg,
-            "step_achieved_gbs": value * b_alg / 1e9}
-    valu = read_profile_table(f"valu_per_kernel_{workload}.txt", dom, tag)
-    if valu is not None:
-        roof["valu_insts"] = valu
-        roof["valu_frac"] = valu / (per_launch[dom] * 1e-3 * CUS * CLOCK_HZ)
+            "frac": achieved / HBM_PEAK_GBS, "scope": "whole step (8 launches, graph replay)",
+            "b_alg_per_update": b_alg, "updates_per_launch": upd_per_step}
+    # the dominant kernel alone: the same algorithmic bytes over its mean
+    # launch time (HIP events on the plan's stream)
+    k_ach = b_alg * upd_per_step / (per_launch[dom] * 1e-3) / 1e9
+    roof["dominant_kernel"] = {"kernel": dom, "ms_per_launch": per_launch[dom], "achieved": k_ach,
+                               "frac": k_ach / HBM_PEAK_GBS}
+    step_kernels = set(per_launch)          # the eight launches of a step (not the per-call refresh)
+    traffic, t_commit = read_profile_table(f"hbm_traffic_per_kernel_{workload}.txt", tag)
+    traffic = {k: v for k, v in traffic.items() if k in step_kernels}
+    if traffic:
+        roof["traffic"] = sum(traffic.values())
+        roof["traffic_source"] = (f"profiles/hbm_traffic_per_kernel_{workload}.txt, rocprofv3 FETCH_SIZE + "
+                                  f"WRITE_SIZE summed over the step's kernels, source commit {t_commit}")
+        roof["traffic_over_alg"] = roof["traffic"] / (b_alg * upd_per_step)
+    else:
+        roof["traffic"] = None
+    valu, v_commit = read_profile_table(f"valu_per_kernel_{workload}.txt", tag)
+    valu = {k: v for k, v in valu.items() if k in step_kernels}
+    if valu:
+        per_step = sum(valu.values())
+        roof["valu"] = {"insts_per_step": per_step,
+                        "lane_ops_per_update": 64.0 * per_step / upd_per_step,
+                        "frac": per_step / (step_s * CUS * CLOCK_HZ),
+                        "dominant_kernel_frac": (valu.get(dom, 0.0) / (per_launch[dom] * 1e-3 * CUS * CLOCK_HZ)),
+                        "peak": "one wave64 VALU instruction per CU per clock (256 CUs, 2.4 GHz)",
+                        "source": f"profiles/valu_per_kernel_{workload}.txt, rocprofv3 SQ_INSTS_VALU, "
+                                  f"source commit {v_commit}"}
     states = plan.states()
     final = [{"macro": [b.macro() for b in s.blocks], "q": [b.Q_D.download() for b in s.blocks],
               "mass": sum(s.total_mass())} for s in sims]
@@ -195,44 +233,70 @@ def errors_between(final_a, final_b):
     return worst
 
 
-def cpu_baseline(gs, a, upd_per_step):
-    """The FP64 oracle on the host cores over a bounded number of steps, and
-    the FP64 GPU step over the same steps from the same initial state."""
+def cpu_sample(gs, workload):
+    """The bounded CPU sample of a workload: the whole case for the single-case
+    workloads, every 16th case of the 64-case batch (4 cases, about 1/16 of
+    its updates; the oracle holds four FP64 f-sized arrays per block)."""
+    if workload.startswith("batch") and len(gs) > 4:
+        idx = list(range(0, len(gs), len(gs) // 4))[:4]
+        return [gs[i] for i in idx], f"cases {idx} of the {len(gs)}-case batch"
+    return gs, "the whole workload"
+
+
+def cpu_baseline(gs, a, workload):
+    """The FP64 oracle on the host CPU over a bounded sample of the workload:
+    all threads of this process's CPU share, then one thread, each the median
+    of three timed runs; then the FP64 GPU step over the same steps from the
+    same initial state, compared with the oracle's state."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     from pyugks_amd.fused import StepPlan
     from pyugks_amd.sim import UGKSim
-    cores = host_cores()
-    print(f"[bench] cpu baseline on {cores} threads", file=sys.stderr, flush=True)
-    oras = [O.OracleSim(copy.deepcopy(g), threads=cores) for g in gs]
-    t0 = time.perf_counter()
-    for o in oras:
-        o.run(1)
-    one = time.perf_counter() - t0
-    n = max(1, int(a.cpu_seconds / max(one, 1e-3)) - 1)
-    # long samples run in chunks of about 30 s, each reported as it ends
-    chunk = max(1, int(30.0 / max(one, 1e-3)))
-    t0 = time.perf_counter()
-    done = 0
-    while done < n:
-        k = min(chunk, n - done)
+    sub, what = cpu_sample(gs, workload)
+    upd = 0
+    for g in sub:
+        gp = copy.deepcopy(g).prepare()
+        upd += sum(b.ni * b.nj for b in gp.blocks) * gp.Nv
+    cores, nproc = host_cores()
+    oras = [O.OracleSim(copy.deepcopy(g), threads=cores) for g in sub]
+    steps_done = 0
+
+    def run(k):
+        nonlocal steps_done
+        t0 = time.perf_counter()
         for o in oras:
             o.run(k)
-        done += k
-        if n > chunk:
-            print(f"[bench] cpu baseline: {done}/{n} steps, {time.perf_counter() - t0:.0f} s",
+        steps_done += k
+        return time.perf_counter() - t0
+
+    def median_rate(threads):
+        O.set_threads(threads)
+        one = run(1)                                   # untimed: sizes the runs
+        k = max(1, int(round(a.cpu_seconds / max(one, 1e-3))))
+        rates = []
+        for r in range(3):
+            dt = run(k)
+            rates.append(upd * k / dt)
+            print(f"[bench] cpu baseline, {threads} thread(s): run {r + 1}/3, {k} step(s) in {dt:.1f} s",
                   file=sys.stderr, flush=True)
-    dt = time.perf_counter() - t0
-    out = {"value": upd_per_step * n / dt, "unit": "cell-velocity updates/s", "cores": cores, "kind": "port",
-           "sample": f"{n} steps of the same workload after 1 untimed step, FP64 C/OpenMP oracle, {dt:.1f} s "
-                     f"on {cores} threads of {cpu_model()}"}
-    sims = [UGKSim(copy.deepcopy(g)) for g in gs]
+        return sorted(rates)[1], rates, k
+
+    print(f"[bench] cpu baseline on {what}: {upd:.3e} updates per step, {cores} threads of {nproc}",
+          file=sys.stderr, flush=True)
+    v_all, r_all, k_all = median_rate(cores)
+    v_one, r_one, k_one = median_rate(1)
+    out = {"value": v_all, "unit": "cell-velocity updates/s", "cores": cores, "kind": "port",
+           "sample": f"{what}; FP64 C/OpenMP oracle; median of 3 runs of {k_all} step(s) on {cores} threads "
+                     f"(the CPU share of this process; the machine has {nproc}); the 1-thread figure is the "
+                     f"median of 3 runs of {k_one} step(s)",
+           "runs": r_all, "one_core": {"value": v_one, "runs": r_one}, "nproc": nproc, "cpu_model": cpu_model()}
+    sims = [UGKSim(copy.deepcopy(g)) for g in sub]
     plan = StepPlan(sims, "fp64")
-    plan.run(n + 1)
+    plan.run(steps_done)
     final_gpu = [{"macro": [b.macro() for b in s.blocks], "q": [b.Q_D.download() for b in s.blocks]} for s in sims]
     final_ora = [{"macro": [b["macro"] for b in o.blocks], "q": [b["Q"] for b in o.blocks]} for o in oras]
     out["fp64_gpu_vs_oracle"] = errors_between(final_gpu, final_ora)
-    out["fp64_gpu_vs_oracle"]["steps"] = n + 1
+    out["fp64_gpu_vs_oracle"]["steps"] = steps_done
     plan.close()
     return out
 
@@ -265,7 +329,7 @@ def main():
     reps, finals = {}, {}
     for p in prec:
         reps[p], finals[p] = run_precision(gs, p, a, upd_per_step, workload)
-    head = prec[0]
+    head = "mixed" if "mixed" in reps else prec[0]
     r = reps[head]
     out = {
         "metric": METRIC,
@@ -278,7 +342,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if head == "fp64" else "f32 storage/flux, f64 moments",
+        "dtype": "f64" if head == "fp64" else "f32 storage/reconstruction/flux, f64 moments/update/dt",
         "data": "synthetic (Maxwellian initial state, seeded perturbations; no input files)",
         "config": {"workload": workload, "cases": len(gs), "updates_per_step": upd_per_step,
                    "Nv": sorted(nv_set), "path": f"fused {head}" + (" graph" if r["graph"] else "")},
@@ -289,16 +353,16 @@ def main():
         "flag": r["flag"],
     }
     if "fp64" in reps and "mixed" in reps:
-        m = reps["mixed"]
-        assert m["steps_done"] == reps["fp64"]["steps_done"]
-        out["mixed"] = {"value": m["value"], "ms_per_step": m["ms_per_step"],
-                        "dtype": "f32 storage/flux, f64 moments", "speedup_vs_fp64": m["value"] / r["value"],
-                        "roofline": m["roofline"], "err_vs_fp64": errors_between(finals["mixed"], finals["fp64"]),
-                        "mass_drift": m["mass_drift"], "kernel_ms_per_launch": m["kernel_ms_per_launch"],
-                        "flag": m["flag"]}
+        f = reps["fp64"]
+        assert f["steps_done"] == r["steps_done"]
+        out["speedup_vs_fp64"] = r["value"] / f["value"]
+        out["err_vs_fp64"] = errors_between(finals["mixed"], finals["fp64"])
+        out["fp64"] = {"value": f["value"], "ms_per_step": f["ms_per_step"], "dtype": "f64",
+                       "roofline": f["roofline"], "mass_drift": f["mass_drift"],
+                       "kernel_ms_per_launch": f["kernel_ms_per_launch"], "flag": f["flag"]}
     finals.clear()
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(gs, a, upd_per_step)
+        out["cpu_baseline"] = cpu_baseline(gs, a, workload)
         out["cpu_baseline"]["gpu_over_cpu"] = r["value"] / out["cpu_baseline"]["value"]
     if any(rep["flag"] for rep in reps.values()):
         # e.g. a WENO5 cavity at CFL 0.8 run past about 75 steps: the scheme

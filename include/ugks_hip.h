@@ -272,6 +272,13 @@ int ugks_step_set_dt(ugks_step_plan *plan, void *stream, double dt);
 int ugks_step_set_dt_case(ugks_step_plan *plan, void *stream, int32_t case_index, double dt);
 /* set dt and hold it for the remaining steps (the last step that lands on max_time) */
 int ugks_step_force_dt_case(ugks_step_plan *plan, void *stream, int32_t case_index, double dt);
+/* 0 if a block of NJ cells per column (ghosts included) and NV velocities can
+ * join a plan of this precision, -2 otherwise: the face passes address the
+ * stencil of a face with 32-bit byte offsets, so (2R-1)*NJ*NV + NV pairs
+ * (R = 3 for WENO5, 2 otherwise) must stay below 2^32 bytes.  Plan creation
+ * applies the same check.  Host only, no device call.  No reference
+ * counterpart (the reference's kernels index f with size_t). */
+int ugks_step_block_fits(int32_t NJ, int32_t NV, int32_t flux_method, int32_t precision);
 /* plain device-to-device copy kernel of `bytes` (a multiple of 16): the
  * reference stream for bandwidth and byte-counter calibration */
 int ugks_copy_kernel(void *stream, const void *src, void *dst, int64_t bytes);

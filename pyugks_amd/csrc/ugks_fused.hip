@@ -281,6 +281,15 @@ __device__ __forceinline__ Stencil<T> stencil(const Consts &C, const void *F, co
     st.sb = (unsigned)(g.step * C.NV * (long long)sizeof(P2<T>));
     return st;
 }
+// The lane offsets rel*sb + vb above are 32-bit: the farthest stencil cell of
+// a W face is 2R-1 rows (NJ cells each) above the base, plus the lane's
+// velocity.  Plans refuse blocks whose span does not fit.
+inline bool stencil_span_fits(long long NJ, long long NV, int method, int precision) {
+    const long long pair = precision == UGKS_FP64 ? 16 : 8;
+    const long long R = method == 1 ? 3 : 2;
+    const long long span = ((2 * R - 1) * NJ + 1) * NV * pair;
+    return span < (1LL << 32);
+}
 template <typename T> using V2 = T __attribute__((ext_vector_type(2)));
 // a store that is not read again soon (per-velocity fluxes, the new f): it
 // streams past the caches instead of evicting stencil data still in use
@@ -1711,6 +1720,7 @@ int ugks_step_plan_create_batch(ugks_block *const *blocks, const ugks_block_buff
         if ((long long)F.nS + (long long)F.nWi * C.nj >= (1LL << 31) - (long long)kTileI * kTileJ * 2) {
             delete p; return ugks_rt::fail(-2, "step plan: too many faces in one block");
         }
+        if (int rc = ugks_step_block_fits(C.NJ, C.NV, C.method, precision)) { delete p; return rc; }
         face_off[b + 1] = face_off[b] + F.nS + (long long)F.nWi * C.nj;
         cell_off[b + 1] = cell_off[b] + (long long)C.ni * C.nj;
         all_off[b + 1] = all_off[b] + (long long)C.NI * C.NJ;
@@ -1953,6 +1963,14 @@ int ugks_step_autotune(ugks_step_plan *p, void *stream, int32_t reps, int32_t *c
     if (p->nfaces == 0) return 0;
     hipStream_t s = (hipStream_t)stream;
     return p->precision == UGKS_FP64 ? autotune<double>(p, s, reps, choice) : autotune<float>(p, s, reps, choice);
+}
+
+int ugks_step_block_fits(int32_t NJ, int32_t NV, int32_t flux_method, int32_t precision) {
+    if (NJ < 1 || NV < 1) return ugks_rt::fail(-1, "block fits: NJ and NV must be positive");
+    if (!stencil_span_fits(NJ, NV, flux_method, precision))
+        return ugks_rt::fail(-2, "step plan: a block's stencil span (2R-1)*NJ*NV pairs exceeds the 32-bit byte "
+                                 "offsets of the face passes; split the block along i");
+    return 0;
 }
 
 int ugks_copy_kernel(void *stream, const void *src, void *dst, int64_t bytes) {

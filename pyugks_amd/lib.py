@@ -75,6 +75,7 @@ FUSED = {
     "ugks_step_set_dt": [P, P, D],
     "ugks_step_set_dt_case": [P, P, I, D],
     "ugks_step_force_dt_case": [P, P, I, D],
+    "ugks_step_block_fits": [I, I, I, I],
     "ugks_copy_kernel": [P, P, P, C.c_int64],
     "ugks_f64_to_f32": [P, P, P, C.c_int64],
     "ugks_f32_to_f64": [P, P, P, C.c_int64],

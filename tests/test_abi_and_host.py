@@ -149,3 +149,24 @@ def test_residual_slope_orders_power_laws():
     assert slopes[2] < 1e-12
     s.step = 5
     assert s.analyse_residual() == np.inf
+
+
+def test_over_wide_block_is_refused_before_any_launch():
+    """The face passes address a face's stencil with 32-bit byte offsets:
+    (2R-1)*NJ*NV + NV pairs must stay below 2^32 bytes.  The check runs on the
+    host (no device call) and plan creation applies it to every block."""
+    L = lib.load()
+    WENO5, VANLEER = 1, 0
+    # the 2x2x128^2 cavity with 33^2 velocities fits easily
+    assert L.ugks_step_block_fits(134, 1089, WENO5, lib.UGKS_FP64) == 0
+    # largest NJ that fits for FP64 WENO5 with 49^2 velocities: (5 NJ + 1) * 2401 * 16 < 2^32
+    nj_max = ((1 << 32) // (2401 * 16) - 1) // 5
+    if ((5 * nj_max + 1) * 2401 * 16) >= (1 << 32):
+        nj_max -= 1
+    assert L.ugks_step_block_fits(nj_max, 2401, WENO5, lib.UGKS_FP64) == 0
+    assert L.ugks_step_block_fits(nj_max + 1, 2401, WENO5, lib.UGKS_FP64) == -2
+    assert b"32-bit" in L.ugks_last_error()
+    # half the bytes per pair in the mixed step, a 3-cell span for vanLeer
+    assert L.ugks_step_block_fits(nj_max + 1, 2401, WENO5, lib.UGKS_MIXED) == 0
+    assert L.ugks_step_block_fits(5 * nj_max // 3, 2401, VANLEER, lib.UGKS_FP64) == 0
+    assert L.ugks_step_block_fits(0, 2401, WENO5, lib.UGKS_FP64) == -1
