@@ -9,9 +9,10 @@
  * (ugks_data.py:215).  OpenMP threads split only the independent outer
  * (cell / face) loops, so results do not depend on the thread count.
  *
- * Parity pins for each part are listed in the header of
- * tests/test_oracle_pins.py.  Functions not covered by a pin there:
- * none on the path (see that file).
+ * Parity pins for each part are listed in the headers of
+ * tests/test_oracle_pins.py, tests/test_oracle_flux_limits.py,
+ * tests/test_oracle_boundaries.py and tests/test_oracle_joins.py.
+ * Functions on the path not covered by one of them: none.
  */
 #include "ugks_oracle.h"
 

@@ -12,22 +12,12 @@
  * the reference does with its default CL_local_size = 1).  The reference
  * file:line each function follows is cited beside it in ugks_oracle.c.
  *
- * Pins: tests/test_oracle_pins.py checks every function against what the
- * mathematics fixes (closed forms, library quadrature, invariants, brute
- * force); the reference ships no golden vectors to check against.  These
- * parts have no such check and are PARITY UNPINNED (they follow a careful
- * reading of the reference only):
- *   - the velocity mirrors (flip_distribution) of or_edgeExchange (its index
- *     maps for all sixteen face pairs, with the orientation flip, are pinned
- *     geometrically; the pure shifts also by the 2x2 split test),
- *   - or_edgeOutflow on north and south faces (east and west are pinned by
- *     the outflow-pressure fixed point),
- *   - or_adsorbingWallS_P2 with sticking (without it, the elastic specular
- *     reflection is pinned; the diffuse and Cercignani-Lampis ones fully),
- *   - the clamp branches of or_adsorbingWall_P1 beyond the mass balance and
- *     coverage bounds that are checked,
- *   - the walk of or_langmuir_table at degenerate edge crossings (the walk
- *     and interpolation are pinned inside the hull and off the table).
+ * Pins: tests/test_oracle_pins.py, tests/test_oracle_flux_limits.py,
+ * tests/test_oracle_boundaries.py and tests/test_oracle_joins.py check every
+ * function against what the mathematics fixes (closed forms, library
+ * quadrature, invariants, limits, brute force); the reference ships no
+ * golden vectors to check against.  Every function on the path has such a
+ * check; none is left parity-unpinned.
  */
 #ifndef UGKS_ORACLE_H
 #define UGKS_ORACLE_H
