@@ -55,7 +55,7 @@ def isotherm_tables(table) -> tuple:
     from scipy.spatial import Delaunay
     tab = np.asarray(table, dtype=np.float64)
     if tab.ndim != 2 or tab.shape[1] != 3:
-        raise RuntimeError("Passed in array for isotherm values is not an n*3 array")
+        raise RuntimeError(f"isotherm table must have three columns (P, T, theta); got shape {tab.shape}")
     d = Delaunay(tab[:, 0:2])
     pts = np.ascontiguousarray([[bake_e15(x) for x in row] for row in tab], dtype=np.float64)
     return (pts, np.ascontiguousarray(d.simplices, dtype=np.int32),
@@ -371,7 +371,7 @@ class UGKSData:
                 else:
                     a = float(bcx.adsorb)
                     if (a > 1.0) | (a <= 0.0):
-                        raise RuntimeError("Boundary condition equilibrium adsorption out of bounds, must be 0 < adsorb <=1")
+                        raise RuntimeError(f"adsorbing wall: equilibrium coverage {a!r} (adsorb / S_T) must lie in (0, 1]")
                     cad.append(1); ads.append(bake_g(a))
             else:
                 bn.append(-1.0); bt.append(-1.0); gf.append(-1.0); ap.append(-1.0)

@@ -37,7 +37,7 @@ def test_descriptor_layouts():
     assert C.sizeof(lib.BlockDesc) == expect
     assert C.sizeof(lib.BlockBuffers) == 21 * 8
     assert C.sizeof(lib.BCDesc) == 6 * 4
-    assert C.sizeof(lib.DtDesc) == 6 * 8 + 4 * 4
+    assert C.sizeof(lib.DtDesc) == 6 * 8 + 4 * 4 + 8
 
 
 def test_missing_library_fails_loudly(tmp_path, monkeypatch):

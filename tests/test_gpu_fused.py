@@ -63,7 +63,10 @@ def test_fused_fp64_matches_oracle(hip, name):
 
 
 @pytest.mark.parametrize("name", ["cavity_vanleer_gh", "cavity_weno_nc_2x2", "cavity_weno_rho_perturbed",
-                                  "channel_adsorbing", "channel_adsorbing_cl", "channel_adsorbing_table",
+                                  "cavity_weno_rho_kn0.075", "cavity_weno_rho_kn1", "cavity_weno_nv1681_rho",
+                                  "cavity_weno_nv2401", "channel_adsorbing", "channel_adsorbing_cl",
+                                  "channel_adsorbing_table", "channel_adsorbing_specular_sticking",
+                                  "north_outflow_south_inflow", "join_self_flips", "join_NS_flipped",
                                   "curvilinear_weno_2x2", "join_N_W", "join_E_E", "cavity_weno_nv1089",
                                   "thin_two_rows", "bounce_back"])
 def test_fused_mixed_error_vs_oracle(hip, name):
@@ -218,3 +221,61 @@ def test_geometry_recomputed_in_place_under_a_plan(hip):
     for xa, xb in zip(a.blocks, b.blocks):
         assert np.array_equal(xa.macro(), xb.macro())
         assert np.array_equal(xa.f(), xb.f())
+
+
+def test_fused_mixed_batch_of_uneven_cases_vs_oracle(hip):
+    """The headline form at small size: eight independent cavities of uneven
+    size, Knudsen number, lid speed and wall temperature, with every
+    velocity grid of the 64-case batch (17^2, 25^2, 33^2 and 41^2), stepped
+    by ONE mixed-precision plan replayed as a HIP graph; after 20 steps each
+    case's rho, U, T are within 1e-5 (L-inf relative) of the FP64 oracle
+    stepping that case alone, and each keeps its own time step."""
+    from oracle import OracleSim
+    from pyugks_amd.fused import StepPlan
+    from pyugks_amd.sim import UGKSim
+    gs = cases.cfg4_batch(ncases=8, sides=(8, 12, 16), vgrids=(16, 24, 32, 40), seed=7)
+    nvs = sorted({(int(g.u_num) // 4) * 4 + 1 for g in gs})
+    print(f"[gpu] mixed batch of {len(gs)} cases, velocity grids {nvs}", flush=True)
+    assert nvs == [17, 25, 33, 41]
+    sims = [UGKSim(copy.deepcopy(g)) for g in gs]
+    plan = StepPlan(sims, "mixed", graph=True)
+    oras = [OracleSim(copy.deepcopy(g), threads=4) for g in gs]
+    nsteps = 20
+    plan.run(nsteps)
+    for o in oras:
+        o.run(nsteps)
+    states = plan.states()
+    assert len({round(d.dt, 12) for d in states}) == len(gs)
+    for i, (s, o, d) in enumerate(zip(sims, oras, states)):
+        assert d.step == o.sim.step
+        assert abs(d.dt - o.dt) <= 1e-5 * o.dt
+        worst = field_errors([b.macro_D.download() for b in s.blocks], [b["macro"] for b in o.blocks])
+        for k in ("rho_linf", "U_linf", "T_linf"):
+            assert worst[k] <= 1e-5, (i, k, worst[k])
+    assert plan.flag() == 0.0
+
+
+def test_fused_fp64_full_size_cavity_2x2x128(hip):
+    """The 2x2-block cavity at its full size (four 128^2 blocks, 33^2
+    velocities, WENO5, Kn 0.01): three FP64 steps of the graph-replayed plan
+    against the oracle on every host thread, f and macro within 1e-12."""
+    import os
+
+    from oracle import OracleSim
+    from pyugks_amd.fused import StepPlan
+    from pyugks_amd.sim import UGKSim
+    g = cases.cfg2_cavity_2x2()
+    print("[gpu] fused fp64 full-size 2x2x128^2 cavity, 33^2 velocities", flush=True)
+    gpu = UGKSim(copy.deepcopy(g))
+    plan = StepPlan(gpu, "fp64", graph=True)
+    ora = OracleSim(copy.deepcopy(g), threads=max(1, min(16, len(os.sched_getaffinity(0)))))
+    for step in range(3):
+        plan.run(1)
+        ora.one_step(True)
+        d = plan.state()
+        assert abs(d.dt - ora.dt) <= TOL * ora.dt
+    for bi, (bg, bo) in enumerate(zip(gpu.blocks, ora.blocks)):
+        assert rel_err(bg.macro_D.download(), bo["macro"]) <= TOL, bi
+        assert rel_err(bg.f_D.download(), bo["f"]) <= TOL, bi
+        assert rel_err(bg.Q_D.download(), bo["Q"], SCALE_FLOOR["Q"]) <= TOL, bi
+    assert plan.flag() == 0.0

@@ -75,3 +75,40 @@ def test_open_domain_mass_monitor(hip):
     assert res[4] > 0                       # |boundary mass flow|: the inflow carries mass in
     bulk, ads = sim.total_mass()
     assert abs(res[5] - abs(sim.initial_mass_total - (bulk + ads)) / sim.initial_mass_total) < 1e-14
+
+
+@pytest.mark.parametrize("fused", [False, True])
+def test_growth_limit_binding_in_the_flux_loop(hip, fused):
+    """dt set to a third of what the blocks allow: the first step computes the
+    flux with dt, re-evaluates (the growth limit doubles it), computes the
+    flux again with 2/3 of the allowance, re-evaluates once more and updates
+    with the full allowance (ugks_sim.py:430-446); later steps grow dt to the
+    limit in turn.  Both the per-kernel loop and the fused plan follow the
+    oracle step for step: dt, time, f and macro within 1e-12."""
+    from oracle import OracleSim
+    from pyugks_amd.fused import StepPlan
+    from pyugks_amd.sim import UGKSim
+    g = cases.cavity(8, ("NC", 8), "WENO5", 0.05, lid=0.2, n_cells_j=6, rho_perturb=0.01)
+    sim = UGKSim(copy.deepcopy(g))
+    ora = OracleSim(copy.deepcopy(g), threads=4)
+    M = sim.gdata.dt
+    assert abs(ora.dt - M) <= 1e-15 * M
+    sim.gdata.dt = M / 3.0
+    ora.sim.dt = M / 3.0
+    plan = StepPlan(sim, "fp64") if fused else None
+    print(f"[gpu] growth limit in the flux loop, fused={fused}", flush=True)
+    for step in range(4):
+        if fused:
+            plan.run(1)
+            d = plan.state()
+            dt, t = d.dt, d.time
+        else:
+            _, err = sim.one_step(True)
+            assert not err
+            dt, t = sim.gdata.dt, sim.gdata.time
+        ora.one_step(True)
+        assert abs(dt - ora.dt) <= 1e-12 * ora.dt, (step, dt, ora.dt)
+        assert abs(t - ora.time) <= 1e-12 * ora.time, (step, t, ora.time)
+        b, o = sim.blocks[0], ora.blocks[0]
+        assert rel_err(b.macro(), o["macro"]) <= 1e-12, step
+        assert rel_err(b.f(), o["f"]) <= 1e-12, step

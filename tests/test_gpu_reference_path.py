@@ -27,6 +27,9 @@ def small_cases():
     out["cavity_weno_nc_2x2"] = cases.cavity(16, ("NC", 12), "WENO5", 0.01, nb=(2, 2), n_cells_j=12)
     out["cavity_minmod_nc"] = cases.cavity(10, ("NC", 8), "minmod", 1.0, lid=0.3, wall_T=1.1)
     out["cavity_weno_rho_perturbed"] = cases.cavity(12, ("NC", 12), "WENO5", 0.001, rho_perturb=0.01)
+    # the 512^2 cavity's other Knudsen numbers, small: tau/dt of order 1 and of order 100
+    out["cavity_weno_rho_kn0.075"] = cases.cavity(12, ("NC", 12), "WENO5", 0.075, rho_perturb=0.01, n_cells_j=10)
+    out["cavity_weno_rho_kn1"] = cases.cavity(10, ("NC", 16), "WENO5", 1.0, rho_perturb=0.01)
     # 33 x 33 velocities: several 256-velocity chunks per wall face, and a 17 x 64 + 1 velocity loop
     out["cavity_weno_nv1089"] = cases.cavity(6, ("NC", 32), "WENO5", 0.01, lid=0.2, n_cells_j=5)
     # a block two cells high between two walls
@@ -64,6 +67,51 @@ def small_cases():
     bcs = [bcd.BounceBackBC(), bcd.ExtrapolateOutBC(), bcd.DiffuseBC(D=1.0, T=0.9, U=0.05), bcd.InflowBC(D=1.2, T=1.0)]
     g.blocks = cartesian_superblock(0.0, 1.0, 0.0, 1.0, 8, 8, bcs, FlowCondition(D=1.0, U=0.3, V=0.0, T=1.0))
     out["bounce_back"] = g
+    # 41 x 41 and 49 x 49 velocities: the wall pass holds 7 and 10 velocity chunks per lane
+    out["cavity_weno_nv1681_rho"] = cases.cavity(6, ("NC", 40), "WENO5", 1.0, lid=0.2, n_cells_j=5, rho_perturb=0.01)
+    out["cavity_weno_nv2401"] = cases.cavity(5, ("NC", 48), "WENO5", 0.075, lid=0.2, n_cells_j=4)
+    # specular adsorbing re-emission with sticking (beta_n, beta_t != 0) on a partly covered wall
+    g = cases._base("WENO5", 0.05)
+    cases.newton_cotes(g, 8)
+    ads = bcd.AdsorbingBC(D=1.0, T=1.1, k_f=0.3, adsorb=0.6, beta_n=0.3, beta_t=0.2, S_T=1.0, cover_initial=0.2,
+                          reflect_type="S")
+    bcs = [bcd.DiffuseBC(D=1.0, T=1.0), bcd.ExtrapolateOutBC(), ads, bcd.InflowBC(D=1.0, T=1.0)]
+    g.blocks = cartesian_superblock(0.0, 2.0, 0.0, 1.0, 14, 6, bcs, FlowCondition(D=1.0, U=0.4, V=0.0, T=1.0), nbj=1,
+                                    i_splits=[6, 8])
+    out["channel_adsorbing_specular_sticking"] = g
+    # the first adsorbing pass's clamp branches (clAdsorb.c:413-449): the coverage cap binds on the
+    # first step (then every impinging velocity sticks: 0/0 in the reflected share); the
+    # desorption clamp binds on the first step and leaves the coverage negative
+    for name, kf, teq in (("channel_adsorbing_cap", 900.0, 0.9), ("channel_adsorbing_desorption_clamp", 300.0, 0.3)):
+        g = cases._base("WENO5", 0.05)
+        cases.newton_cotes(g, 8)
+        ads = bcd.AdsorbingBC(D=1.0, T=1.0, k_f=kf, adsorb=teq, S_T=1.0, cover_initial=0.999, reflect_type="D")
+        bcs = [bcd.DiffuseBC(D=1.0, T=1.0), bcd.ExtrapolateOutBC(), ads, bcd.InflowBC(D=1.0, T=1.0)]
+        g.blocks = cartesian_superblock(0.0, 1.0, 0.0, 1.0, 8, 6, bcs, FlowCondition(D=1.0, U=0.4, V=0.0, T=1.0))
+        out[name] = g
+    # outflow on a north face and inflow on a south face (ghost velocity in the face frame)
+    g = cases._base("WENO5", 0.05)
+    cases.newton_cotes(g, 8)
+    bcs = [bcd.OutflowBC(P=0.9, D=1.0), bcd.DiffuseBC(D=1.0, T=1.0), bcd.InflowBC(D=1.1, T=1.05),
+           bcd.DiffuseBC(D=1.0, T=1.0)]
+    g.blocks = cartesian_superblock(0.0, 1.0, 0.0, 1.0, 7, 9, bcs, FlowCondition(D=1.0, U=0.05, V=0.2, T=1.0))
+    out["north_outflow_south_inflow"] = g
+    # velocity flips of joins: faces joined to themselves (FLIP_NS on north, FLIP_D on west) ...
+    g = cases._base("WENO5", 0.05)
+    cases.newton_cotes(g, 8)
+    bcs = [bcd.PeriodicBC(0, bcd.NORTH, flip_distribution=bcd.FLIP_NS), bcd.DiffuseBC(D=1.0, T=1.0),
+           bcd.DiffuseBC(D=1.0, T=0.9, U=0.2), bcd.PeriodicBC(0, bcd.WEST, flip_distribution=bcd.FLIP_D)]
+    g.blocks = cartesian_superblock(0.0, 1.0, 0.0, 1.0, 8, 6, bcs, FlowCondition(D=1.0, T=1.0))
+    out["join_self_flips"] = g
+    # ... and two blocks joined north-south with FLIP_NS (east-west with FLIP_EW)
+    for name, nb, flip in (("join_NS_flipped", (1, 2), bcd.FLIP_NS), ("join_EW_flipped", (2, 1), bcd.FLIP_EW)):
+        g = cases.cavity(8, ("NC", 8), "vanLeer" if flip == bcd.FLIP_EW else "WENO5", 0.05, lid=0.2, n_cells_j=10,
+                         nb=nb)
+        for b in g.blocks:
+            for bcx in b.bc_list:
+                if bcx.type_of_BC == bcd.ADJACENT:
+                    bcx.flip_distribution = flip
+        out[name] = g
     return out
 
 
