@@ -55,6 +55,16 @@ typedef struct ugks_block_desc {
     const double *iso[4];
     const int32_t *iso_tri[4], *iso_nbr[4];
     int32_t n_iso[4], n_tri[4];
+    /* Wall values given as expressions (UDF_D/U/V/T, geom_bc_defs.py:115-146, which
+     * ugks_CL.py:278-298 pastes into the kernel source): stack programs in s, x, y, t
+     * compiled by the host (opcodes in pyugks_amd/udf.py).  Slot 4*w + q (wall w = N,E,S,W;
+     * q = 0 D, 1 U, 2 V, 3 T, the temperature itself) runs udf_len[slot] opcodes from
+     * udf_code + udf_start[slot], with constants from udf_const; udf_len 0 keeps the
+     * constant wall value.  ugks_block_create checks every program before it is copied. */
+    const int32_t *udf_code;
+    const double *udf_const;
+    int32_t udf_ncode, udf_nconst;
+    int32_t udf_start[16], udf_len[16];
 } ugks_block_desc;
 
 typedef struct ugks_block ugks_block;
@@ -217,6 +227,8 @@ typedef struct ugks_block_buffers {
     void *macro, *macro_new, *residual, *Q;
     const void *xy, *centre, *mid_side, *normal, *area, *side_length;
     void *wall_prop, *wall_cover, *wall_fluxes, *wall_dist, *time_step, *flag;
+    const void *para;   /* [wall_len][4] double4 (s, x, y, 0) of each wall cell (ugks_block.py:776-836);
+                           read when the block's wall values depend on time */
 } ugks_block_buffers;
 
 /* One face's BC (geom/geom_bc_defs.py:27-51): type code, partner block and
@@ -228,10 +240,15 @@ typedef struct ugks_bc_desc {
 
 /* Time-step state (the gdata fields of ugks_sim.py:285-311, :491); overshoot
  * reports a first step whose re-evaluated dt differed (ugks_sim.py:443-445);
- * force_time holds dt fixed (UGKSim.force_time, ugks_sim.py:292, :673-675). */
+ * force_time holds dt fixed (UGKSim.force_time, ugks_sim.py:292, :673-675).
+ * max_dt is the smallest block maximum time step of the case's last
+ * calcTimeStep (UGKSBlock.max_dt, ugks_block.py:1415-1433), against which the
+ * first step re-evaluates dt as UGKSim.one_step does (ugks_sim.py:430-446);
+ * 0 takes dt as it is.  A plan reports it back as the smallest of its blocks. */
 typedef struct ugks_dt_desc {
     double dt, time, CFL, delta_dt, suggest_dt, Kn_eff;
     int32_t step, overshoot, force_time, pad;
+    double max_dt;
 } ugks_dt_desc;
 
 typedef struct ugks_step_plan ugks_step_plan;
@@ -285,6 +302,7 @@ int ugks_copy_kernel(void *stream, const void *src, void *dst, int64_t bytes);
 /* element-wise precision conversion of n values (f of the mixed-precision step) */
 int ugks_f64_to_f32(void *stream, const void *src, void *dst, int64_t n);
 int ugks_f32_to_f64(void *stream, const void *src, void *dst, int64_t n);
+
 
 #ifdef __cplusplus
 }

@@ -82,6 +82,11 @@ def lib():
     if _lib is None:
         _lib = C.CDLL(build())
         _lib.or_sim_init.argtypes = [C.POINTER(Sim)]
+        _lib.or_block_init.argtypes = [C.POINTER(Block)]
+        _lib.or_sim_updateAllBC.argtypes = [C.POINTER(Sim)]
+        _lib.or_sim_calcTimeStep.argtypes = [C.POINTER(Sim)]
+        _lib.or_sim_updateTimeStep.argtypes = [C.POINTER(Sim), C.c_int]
+        _lib.or_cellGeom.argtypes = [C.c_void_p] * 7
         _lib.or_sim_one_step.argtypes = [C.POINTER(Sim), C.c_int]
         _lib.or_sim_run.argtypes = [C.POINTER(Sim), C.c_int]
         _lib.or_set_threads.argtypes = [C.c_int]
@@ -192,6 +197,68 @@ class OracleBlock:
         return self.arrays[name]
 
 
+# ---- wall values given as expressions (UDF_D/U/V/T), restated ------------
+# The reference pastes each UDF string into the OpenCL source as a #define
+# (ugks_CL.py:278-298) and paraBC evaluates it per wall cell in s, x, y, t
+# (clKernel.c:141-218).  Here the string is walked as a C expression tree with
+# numpy: integer literals stay integers (C division and remainder truncate
+# toward zero), anything touching a double is double; PI is '%0.15e' % pi.
+_UDF_FUNCS = {"sin": np.sin, "cos": np.cos, "tan": np.tan, "exp": np.exp, "log": np.log, "sqrt": np.sqrt,
+              "fabs": np.abs, "tanh": np.tanh, "atan": np.arctan, "asin": np.arcsin, "acos": np.arccos,
+              "sinh": np.sinh, "cosh": np.cosh, "log10": np.log10, "floor": np.floor, "ceil": np.ceil,
+              "pow": np.power, "fmin": np.fmin, "fmax": np.fmax, "atan2": np.arctan2, "fmod": np.fmod}
+
+
+def udf_values(text: str, s, x, y, t):
+    """The value of a UDF string at wall cells (s, x, y arrays) and time t."""
+    import ast
+
+    def ev(n):
+        if isinstance(n, ast.Expression):
+            return ev(n.body)
+        if isinstance(n, ast.Constant):
+            return n.value, isinstance(n.value, int)
+        if isinstance(n, ast.Name):
+            env = {"s": s, "x": x, "y": y, "t": t, "PI": 3.141592653589793, "M_PI": 3.141592653589793}
+            return np.asarray(env[n.id], dtype=np.float64), False
+        if isinstance(n, ast.UnaryOp):
+            v, k = ev(n.operand)
+            return (-v if isinstance(n.op, ast.USub) else v), k
+        if isinstance(n, ast.BinOp):
+            (a, ka), (b, kb) = ev(n.left), ev(n.right)
+            if ka and kb:
+                if isinstance(n.op, (ast.Div, ast.Mod)):
+                    q = abs(a) // abs(b) * (1 if (a >= 0) == (b >= 0) else -1)
+                    return (q if isinstance(n.op, ast.Div) else a - q * b), True
+                return {ast.Add: a + b, ast.Sub: a - b, ast.Mult: a * b}[type(n.op)], True
+            a, b = np.asarray(a, dtype=np.float64), np.asarray(b, dtype=np.float64)
+            with np.errstate(all="ignore"):
+                return {ast.Add: lambda: a + b, ast.Sub: lambda: a - b, ast.Mult: lambda: a * b,
+                        ast.Div: lambda: a / b}[type(n.op)](), False
+        if isinstance(n, ast.Call):
+            args = [np.asarray(ev(a)[0], dtype=np.float64) for a in n.args]
+            with np.errstate(all="ignore"):
+                return _UDF_FUNCS[n.func.id](*args), False
+        raise ValueError(f"unsupported UDF syntax: {ast.dump(n)[:60]}")
+
+    return np.broadcast_to(np.float64(ev(ast.parse(text.strip(), mode="eval"))[0]), np.shape(s)).astype(np.float64)
+
+
+def wall_para(ni, nj, G, side_length, x, y):
+    """(s, x, y) of every wall cell, [max(ni, nj)][4][3] (ugks_block.py:776-836):
+    s the cell midpoint's arc length over the wall length, (x, y) the mean of
+    its two vertices, the west wall's taken from vertex column 1 (:786)."""
+    out = np.zeros((max(ni, nj), 4, 3))
+    L = side_length
+    for w, lens, xs, ys in ((0, L[G:G + ni, G + nj, 0], x[:, nj], y[:, nj]), (1, L[G + ni, G:G + nj, 1], x[ni, :], y[ni, :]),
+                            (2, L[G:G + ni, G, 0], x[:, 0], y[:, 0]), (3, L[G, G:G + nj, 1], x[1, :], y[1, :])):
+        cum = np.concatenate([[0.0], np.cumsum(lens)])
+        out[:len(lens), w, 0] = (cum[:-1] + 0.5 * lens) / cum[-1]
+        out[:len(lens), w, 1] = 0.5 * (xs[:-1] + xs[1:])
+        out[:len(lens), w, 2] = 0.5 * (ys[:-1] + ys[1:])
+    return out
+
+
 class OracleSim:
     """The reference sequence (UGKSim.__init__ step part, one_step, run) in C."""
 
@@ -220,19 +287,82 @@ class OracleSim:
         self.sim.force_time = 0
         self.sim.dt_update_count = gdata.dt_update_count
         self.sim.max_dt = self.max_dt.ctypes.data
-        rc = lib().or_sim_init(C.byref(self.sim))
-        if rc:
-            raise RuntimeError(f"oracle init failed {rc}")
+        # UGKSim.__init__ from initFunctions on (ugks_sim.py:81-92), with the
+        # wall-value expressions evaluated where paraBC runs
+        L = lib()
+        self.udf = [[bc if getattr(bc, "UDF_D", None) is not None or getattr(bc, "UDF_U", None) is not None
+                     or getattr(bc, "UDF_V", None) is not None or getattr(bc, "UDF_T", None) is not None else None
+                     for bc in bd.bc_list] for bd in gdata.blocks]
+        self.time_dependent = [any(u is not None and "t" in u for bc in bd.bc_list
+                                   for u in (getattr(bc, "UDF_D", None), getattr(bc, "UDF_T", None),
+                                             getattr(bc, "UDF_U", None), getattr(bc, "UDF_V", None)))
+                               for bd in gdata.blocks]
+        for b in self.blocks:
+            a = b.arrays
+            L.or_cellGeom(C.byref(b.block.p), *[C.c_void_p(a[k].ctypes.data) for k in
+                                                 ("xy", "area", "centre", "mid_side", "normal", "side_length")])
+        for bi, b in enumerate(self.blocks):
+            L.or_block_init(C.byref(self._barr[bi]))
+            self._para_udf(bi)
+        L.or_sim_updateAllBC(C.byref(self.sim))
+        L.or_sim_calcTimeStep(C.byref(self.sim))
+        L.or_sim_updateTimeStep(C.byref(self.sim), 0)
+
+    def _para_udf(self, bi):
+        """paraBC on the walls with UDF strings (clKernel.c:141-218): D, U, V
+        and T from the expressions (or the constants), s3 = P / D with a
+        pressure, else 1 / T."""
+        bcs = self.udf[bi]
+        if not any(x is not None for x in bcs):
+            return
+        b, bd = self.blocks[bi], self.gdata.blocks[bi]
+        c = b.consts
+        ni, nj, G = c["ni"], c["nj"], c["ghost"]
+        para = wall_para(ni, nj, G, b["side_length"], np.asarray(bd.x, dtype=np.float64),
+                         np.asarray(bd.y, dtype=np.float64))
+        t = self.sim.time
+        for w, bc in enumerate(bcs):
+            if bc is None or bc.type_of_BC not in (2, 4, 6, 7, 8):     # CONSTANT DIFFUSE INFLOW OUTFLOW ADSORBING
+                continue
+            n = ni if w in (0, 2) else nj
+            s_, x_, y_ = para[:n, w, 0], para[:n, w, 1], para[:n, w, 2]
+
+            def val(name, k):
+                text = getattr(bc, name, None)
+                return udf_values(text, s_, x_, y_, t) if text is not None else np.full(n, c[k][w])
+            D_ = val("UDF_D", "wall_D")
+            wp = b["wall_prop"]
+            wp[:n, w, 0] = D_
+            wp[:n, w, 1] = val("UDF_U", "wall_U")
+            wp[:n, w, 2] = val("UDF_V", "wall_V")
+            if c["wall_P"][w] != -1:
+                wp[:n, w, 3] = c["wall_P"][w] / D_
+            elif bc.UDF_T is not None:
+                wp[:n, w, 3] = 1.0 / udf_values(bc.UDF_T, s_, x_, y_, t)
+            else:
+                wp[:n, w, 3] = c["wall_T"][w]
 
     def one_step(self, get_dt=True):
+        for bi, td in enumerate(self.time_dependent):
+            if td:
+                self._para_udf(bi)          # UGKSBlock.parametricBC at the step's start time
         rc = lib().or_sim_one_step(C.byref(self.sim), int(bool(get_dt)))
         if rc:
             raise RuntimeError(f"oracle step failed {rc}")
 
     def run(self, nsteps: int):
+        if any(self.time_dependent):
+            for _ in range(int(nsteps)):
+                self.one_step(self._get_dt())
+            return
         rc = lib().or_sim_run(C.byref(self.sim), int(nsteps))
         if rc:
             raise RuntimeError(f"oracle run failed {rc}")
+
+    def _get_dt(self):
+        """or_sim_run's dt cadence (ugks_sim.py:542-578)."""
+        n, st = self.sim.dt_update_count, self.sim.step
+        return (st == -n) if n < 0 else ((st + 1) % n == 0)
 
     @property
     def dt(self):

@@ -40,9 +40,11 @@ class BoundaryCondition:
                  adsorb=None, beta_n=0, beta_t=0, alpha_n=1, alpha_t=1, k_f=0,
                  S_T=1.0, cover_initial=0, reflect_type="S", dynamic=1,
                  other_block=-1, other_face=-1, orientation=NO_HOLD,
-                 flip_distribution=NO_FLIP, label=""):
+                 flip_distribution=NO_FLIP, label="", UDF_D=None, UDF_U=None, UDF_V=None, UDF_T=None):
         self.type_of_BC = type_of_BC
         self.D, self.T, self.U, self.V, self.P = D, T, U, V, P
+        # wall values as OpenCL C expressions in s, x, y, t (geom_bc_defs.py:115-146)
+        self.UDF_D, self.UDF_U, self.UDF_V, self.UDF_T = UDF_D, UDF_U, UDF_V, UDF_T
         self.adsorb = copy.copy(adsorb)
         self.beta_n, self.beta_t = beta_n, beta_t
         self.alpha_n, self.alpha_t = alpha_n, alpha_t
@@ -56,6 +58,9 @@ class BoundaryCondition:
         self.orientation = orientation
         self.flip_distribution = flip_distribution
         self.label = label
+
+    def has_udf(self) -> bool:
+        return any(u is not None for u in (self.UDF_D, self.UDF_U, self.UDF_V, self.UDF_T))
 
     def __repr__(self):
         return f"BC(type={self.type_of_BC}, D={self.D}, U={self.U}, V={self.V}, T={self.T}, label={self.label!r})"
@@ -79,9 +84,10 @@ def ExtrapolateOutBC():
     return BoundaryCondition(EXTRAPOLATE_OUT, label="EXTRAPOLATE OUT")
 
 
-def ConstantBC(D=0, T=0, U=0, V=0):
+def ConstantBC(D=0, T=0, U=0, V=0, UDF_D=None, UDF_U=None, UDF_V=None, UDF_T=None):
     """geom_bc_defs.py:257-286"""
-    return BoundaryCondition(CONSTANT, D=D, T=T, U=U, V=V, label="CONSTANT")
+    return BoundaryCondition(CONSTANT, D=D, T=T, U=U, V=V, label="CONSTANT", UDF_D=UDF_D, UDF_U=UDF_U,
+                             UDF_V=UDF_V, UDF_T=UDF_T)
 
 
 def ReflectBC():
@@ -94,18 +100,21 @@ def BounceBackBC():
     return BoundaryCondition(BOUNCE_BACK, label="BOUNCE_BACK")
 
 
-def DiffuseBC(D=1, T=0, U=0, V=0):
+def DiffuseBC(D=1, T=0, U=0, V=0, UDF_D=None, UDF_U=None, UDF_V=None, UDF_T=None):
     """geom_bc_defs.py:316-353"""
-    return BoundaryCondition(DIFFUSE, D=D, T=T, U=U, V=V, label="DIFFUSE")
+    return BoundaryCondition(DIFFUSE, D=D, T=T, U=U, V=V, label="DIFFUSE", UDF_D=UDF_D, UDF_U=UDF_U, UDF_V=UDF_V,
+                             UDF_T=UDF_T)
 
 
 def AdsorbingBC(D=1, T=0, U=0, V=0, adsorb=None, beta_n=0, beta_t=0, alpha_n=1,
-                alpha_t=1, k_f=0, S_T=1.0, cover_initial=0, reflect_type="S", dynamic=1):
+                alpha_t=1, k_f=0, S_T=1.0, cover_initial=0, reflect_type="S", dynamic=1,
+                UDF_D=None, UDF_U=None, UDF_V=None, UDF_T=None):
     """geom_bc_defs.py:355-414"""
     return BoundaryCondition(ADSORBING, D=D, T=T, U=U, V=V, adsorb=adsorb, beta_n=beta_n,
                              beta_t=beta_t, alpha_n=alpha_n, alpha_t=alpha_t, k_f=k_f,
                              S_T=S_T, cover_initial=cover_initial, reflect_type=reflect_type,
-                             dynamic=dynamic, label="ADSORBING")
+                             dynamic=dynamic, label="ADSORBING", UDF_D=UDF_D, UDF_U=UDF_U, UDF_V=UDF_V,
+                             UDF_T=UDF_T)
 
 
 def InflowBC(D=1, T=1):

@@ -13,6 +13,7 @@ import numpy as np
 from . import bc as bcd
 from . import lib as L
 from .data import UGKSData, BlockDef, initial_macro
+from .udf import para_table
 
 SOUTH, WEST = np.int32(0), np.int32(1)
 
@@ -66,6 +67,10 @@ class UGKSBlock:
         self.face_lengths = [length_H[G:-G, -G, 0], length_H[-G, G:-G, 1],
                              length_H[G:-G, G, 0], length_H[G, G:-G, 1]]
         self.total_area = float(np.sum(self.area_H[G:-G, G:-G]))
+        # the parametric position of every wall cell, read by wall-value expressions (ugks_block.py:776-836)
+        self.para_H = para_table(self.ni, self.nj, G, length_H, np.asarray(self.bdef.x, dtype=np.float64),
+                                 np.asarray(self.bdef.y, dtype=np.float64))
+        self.para_D.upload(self.para_H, self.stream)
 
     def initFunctions(self, restart=None):
         """ugks_block.py:258-432.  ``restart``: this block's record from
@@ -121,8 +126,9 @@ class UGKSBlock:
 
     # ------------------------------------------------------------- BCs
     def parametricBC(self, force=False):
-        """ugks_block.py:603-619 (constant wall values; no time-dependent UDFs)."""
-        if force:
+        """ugks_block.py:603-619: the wall values at the current time, every
+        step when a UDF string of the block contains the letter t."""
+        if force or self.consts["time_dependent_BC"]:
             self.prg.paraBC(self.stream, self.para_D, self.wall_prop_D, np.float64(self.gdata.time))
 
     def ghostExchange(self, this_face, other_block, other_face, ori=1, flip=bcd.NO_FLIP):

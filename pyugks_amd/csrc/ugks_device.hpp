@@ -54,7 +54,81 @@ struct Consts {
     // so a Maxwellian factorises into per-axis exponentials (nut = 0 if not a tensor grid)
     int nut, nvt;
     const double *qu, *qv;
+    // wall values given as expressions (UDF_*): slot 4*w + q, q = 0 D, 1 U, 2 V, 3 T;
+    // udf_len 0 keeps the constant wall value; any_udf / time_udf summarise the block
+    const int *udf_code;
+    const double *udf_const;
+    int udf_start[16], udf_len[16];
+    int any_udf, time_udf;
 };
+
+// ---- wall-value expressions (paraBC with UDF text, clKernel.c:141-218) ----
+// The stack program of one slot, in the wall cell's parametric position s,
+// midpoint (x, y) and time t (opcodes: pyugks_amd/udf.py; every program is
+// checked on the host to stay within 16 stack entries and its tables).
+__device__ inline double udf_eval(const Consts &C, int slot, double s, double x, double y, double t) {
+    const int *c = C.udf_code + C.udf_start[slot];
+    const int n = C.udf_len[slot];
+    double st[16];
+    int sp = 0;
+    for (int i = 0; i < n; i++) {
+        const int op = c[i];
+        if (op == 1) { st[sp++] = C.udf_const[c[++i]]; continue; }
+        if (op >= 2 && op <= 5) { st[sp++] = op == 2 ? s : op == 3 ? x : op == 4 ? y : t; continue; }
+        if ((op >= 10 && op <= 15 && op != 14) || op >= 40) {
+            const double b = st[--sp], a = st[sp - 1];
+            double r;
+            switch (op) {
+            case 10: r = a + b; break;
+            case 11: r = a - b; break;
+            case 12: r = a * b; break;
+            case 13: r = a / b; break;
+            case 15: r = pow(a, b); break;
+            case 40: r = fmin(a, b); break;
+            case 41: r = fmax(a, b); break;
+            case 42: r = atan2(a, b); break;
+            default: r = fmod(a, b); break;
+            }
+            st[sp - 1] = r;
+            continue;
+        }
+        const double a = st[sp - 1];
+        double r;
+        switch (op) {
+        case 14: r = -a; break;
+        case 20: r = sin(a); break;
+        case 21: r = cos(a); break;
+        case 22: r = tan(a); break;
+        case 23: r = exp(a); break;
+        case 24: r = log(a); break;
+        case 25: r = sqrt(a); break;
+        case 26: r = fabs(a); break;
+        case 27: r = tanh(a); break;
+        case 28: r = atan(a); break;
+        case 29: r = asin(a); break;
+        case 30: r = acos(a); break;
+        case 31: r = sinh(a); break;
+        case 32: r = cosh(a); break;
+        case 33: r = log10(a); break;
+        case 34: r = floor(a); break;
+        default: r = ceil(a); break;
+        }
+        st[sp - 1] = r;
+    }
+    return st[0];
+}
+// paraBC for wall w at one wall cell: the WALL_* values, an expression where
+// the wall has one; s3 = P / D where a pressure is set, else 1 / T
+__device__ inline d4 para_wall(const Consts &C, int w, d4 pa, double t) {
+    auto val = [&](int q, double k) { return C.udf_len[4 * w + q] ? udf_eval(C, 4 * w + q, pa.s0, pa.s1, pa.s2, t) : k; };
+    d4 p;
+    p.s0 = val(0, C.wall_D[w]);
+    p.s1 = val(1, C.wall_U[w]);
+    p.s2 = val(2, C.wall_V[w]);
+    if (C.wall_P[w] != -1) p.s3 = C.wall_P[w] / p.s0;
+    else p.s3 = C.udf_len[4 * w + 3] ? 1.0 / udf_eval(C, 4 * w + 3, pa.s0, pa.s1, pa.s2, t) : C.wall_T[w];
+    return p;
+}
 
 // ---- indexing of the reference buffers (clKernel.c:4-26) ------------------
 __device__ __forceinline__ size_t fidx(const Consts &C, int i, int j, int v) {
@@ -403,6 +477,69 @@ __device__ __forceinline__ double wave_sum(double x) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
     return x;
+}
+
+// Multi-value wavefront sums without LDS traffic.  A "row" is 16 lanes.
+// v_permlane32_swap exchanges the upper half of its first operand with the
+// lower half of its second, v_permlane16_swap the odd rows of the first with
+// the even rows of the second (gfx950); after either, the lane-wise sum of the
+// two operands holds, in each half (row pair), one of the two inputs summed
+// over both halves (rows): a reduce-scatter step that moves no data through
+// LDS.  The last four steps add the row rotated by 8, 4, 2 and 1 lanes (DPP);
+// every lane of a row then holds the row total, bit for bit the same in each
+// lane (each step adds the same two operands in both lanes of a pair).
+__device__ __forceinline__ double pack_d(unsigned lo, unsigned hi) {
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ __forceinline__ unsigned lo_d(double x) { return (unsigned)(unsigned long long)__double_as_longlong(x); }
+__device__ __forceinline__ unsigned hi_d(double x) { return (unsigned)((unsigned long long)__double_as_longlong(x) >> 32); }
+// x + y after swapping x's upper 32 lanes with y's lower 32: lanes 0-31 hold
+// x summed over both halves, lanes 32-63 y summed over both halves
+__device__ __forceinline__ double fold32(double x, double y) {
+    const auto l = __builtin_amdgcn_permlane32_swap(lo_d(x), lo_d(y), false, false);
+    const auto h = __builtin_amdgcn_permlane32_swap(hi_d(x), hi_d(y), false, false);
+    return pack_d(l[0], h[0]) + pack_d(l[1], h[1]);
+}
+// the same over rows: rows 0 and 2 hold x summed over rows (0, 1) and (2, 3),
+// rows 1 and 3 hold y likewise
+__device__ __forceinline__ double fold16(double x, double y) {
+    const auto l = __builtin_amdgcn_permlane16_swap(lo_d(x), lo_d(y), false, false);
+    const auto h = __builtin_amdgcn_permlane16_swap(hi_d(x), hi_d(y), false, false);
+    return pack_d(l[0], h[0]) + pack_d(l[1], h[1]);
+}
+template <int CTRL> __device__ __forceinline__ double dpp_d(double x) {
+    return pack_d((unsigned)__builtin_amdgcn_mov_dpp((int)lo_d(x), CTRL, 0xf, 0xf, false),
+                  (unsigned)__builtin_amdgcn_mov_dpp((int)hi_d(x), CTRL, 0xf, 0xf, false));
+}
+__device__ __forceinline__ double row_sum(double x) {
+    x += dpp_d<0x128>(x);   // row_ror:8
+    x += dpp_d<0x124>(x);   // row_ror:4
+    x += dpp_d<0x122>(x);   // row_ror:2
+    x += dpp_d<0x121>(x);   // row_ror:1
+    return x;
+}
+// the double in lane l (l wavefront-uniform), to every lane
+__device__ __forceinline__ double lane_d(double x, int l) {
+    return pack_d((unsigned)__builtin_amdgcn_readlane((int)lo_d(x), l), (unsigned)__builtin_amdgcn_readlane((int)hi_d(x), l));
+}
+// Totals of a[0..3] over the wavefront: every lane of row r gets the total of a[r].
+__device__ __forceinline__ double wave_sum4_rows(const double (&a)[4]) {
+    return row_sum(fold16(fold32(a[0], a[2]), fold32(a[1], a[3])));
+}
+// Totals of a[0..3], to every lane (gathered from the rows).
+__device__ __forceinline__ void wave_sum4(double (&a)[4]) {
+    const double r = wave_sum4_rows(a);
+#pragma unroll
+    for (int q = 0; q < 4; q++) a[q] = lane_d(r, 16 * q);
+}
+// Totals of a[0..15] over the wavefront: every lane of row r gets the totals
+// of a[4r .. 4r+3] in t[0..3].
+__device__ __forceinline__ void wave_sum16_rows(const double (&a)[16], double (&t)[4]) {
+    double s[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) s[k] = fold32(a[k], a[k + 8]);      // value k (lanes 0-31), k + 8 (32-63)
+#pragma unroll
+    for (int k = 0; k < 4; k++) t[k] = row_sum(fold16(s[k], s[k + 4]));   // value k + 4 r in row r
 }
 
 }  // namespace ugks

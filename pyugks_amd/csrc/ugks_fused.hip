@@ -36,6 +36,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "../../include/ugks_hip.h"
@@ -64,6 +65,7 @@ struct FBlock {
     const double *area, *side_length;
     const d4 *wall_prop;
     double *wall_fluxes, *time_step, *flag;
+    const d4 *para;                         // [wl][4] (s, x, y, 0) of each wall cell
     int bc[4], other[4], oface[4], ori[4], flip[4], reflect[4];
     int nS, nSj, ob_S, ob_W, nWi, wl;       // interior-face counts and offsets
     int flipidx[4];                         // exchange row flip from the corner check
@@ -71,10 +73,20 @@ struct FBlock {
     d4 *cmom;                               // [NI][NJ] moments of f per cell (all cells, ghosts included)
 };
 
+// A case's time-step state.  dt is gdata.dt as the host sees it after a step;
+// dt_flux and dt_up are the time steps the next step's flux and update use.
+// They differ from dt when the growth limit binds as the step re-evaluates
+// dt: the reference computes the flux with dt, re-evaluates, and if dt moved
+// computes the flux again with the new value and re-evaluates once more
+// before the update (ugks_sim.py:430-446).
 struct DtState {
-    double dt, time, CFL, delta_dt, suggest_dt, Kn;
+    double dt, dt_flux, dt_up, time, CFL, delta_dt, suggest_dt, Kn;
     int step, overshoot, force_time, pad;
 };
+// updateTimeStep (ugks_sim.py:285-311) towards the block minimum M
+__host__ __device__ inline double limited_dt(double cur, double M, double lim) {
+    return cur != M ? (M > lim * cur ? lim * cur : M) : cur;
+}
 
 // Workgroups are dealt to the 8 XCDs round robin (workgroup id % 8), and
 // each XCD has its own L2.  Neighbouring faces and cells share f stencils and
@@ -573,27 +585,73 @@ __device__ __forceinline__ FaceLoc face_loc_tiled(const FBlock &B, long long loc
     return f;
 }
 
+// One interior face as the face passes see it (64 bytes, one scalar load):
+// its block, side (SOUTH / WEST), owner cell and the reconstruction
+// distances of both upwind choices.  The plan fills the table from the mesh
+// (k_facerec) at every run call, so a geometry update under a live plan is
+// picked up; the face passes then neither search the block offsets nor
+// recompute distances per face.
+struct __align__(64) FaceRec {
+    int b, face, gi, gj;
+    double dist[2], rdist[2], rd1[2];
+};
+static_assert(sizeof(FaceRec) == 64, "face record is one 64-byte line");
+
+__device__ __forceinline__ FaceGeo face_geo_of(const FaceRec &r, const Consts &C) {
+    FaceGeo g;
+    g.own = (size_t)r.gi * C.NJ + r.gj;
+    g.step = r.face == SOUTH ? 1 : C.NJ;
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        g.dist[k] = r.dist[k];
+        g.rdist[k] = r.rdist[k];
+        g.rd1[k] = r.rd1[k];
+    }
+    return g;
+}
+
+// rec[fid]: the face of plain index fid (per block, S faces then W faces);
+// tmap[fid]: the plain index of the moment pass's item fid (tiled order)
+__global__ void __launch_bounds__(kT) k_facerec(const FBlock *__restrict__ blocks, const long long *__restrict__ off,
+                                                int nb, long long nfaces, FaceRec *__restrict__ rec,
+                                                int *__restrict__ tmap) {
+    const long long fid = (long long)blockIdx.x * kT + threadIdx.x;
+    if (fid >= nfaces) return;
+    const int b = find_block(off, nb, fid);
+    const FBlock &B = blocks[b];
+    const long long loc = fid - off[b];
+    const FaceLoc L = face_loc(B, loc);
+    const FaceGeo g = face_geo(B, L.gi, L.gj, L.face);
+    FaceRec r;
+    r.b = b; r.face = L.face; r.gi = L.gi; r.gj = L.gj;
+#pragma unroll
+    for (int k = 0; k < 2; k++) { r.dist[k] = g.dist[k]; r.rdist[k] = g.rdist[k]; r.rd1[k] = g.rd1[k]; }
+    rec[fid] = r;
+    tmap[fid] = (int)(off[b] + face_index(B, face_loc_tiled(B, loc)));
+}
+
 template <typename T, int FW, int M>
-__global__ void __launch_bounds__(64 * FW) k_faceA(const FBlock *__restrict__ blocks, const long long *__restrict__ off,
-                                                   int nb, long long nfaces, double *__restrict__ fsum) {
+__global__ void __launch_bounds__(64 * FW) k_faceA(const FBlock *__restrict__ blocks, const FaceRec *__restrict__ recs,
+                                                   const int *__restrict__ tmap, long long nfaces,
+                                                   double *__restrict__ fsum) {
     const long long fid = xcd_remap(blockIdx.x, gridDim.x) * FW + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (fid >= nfaces) return;
     const int lane = threadIdx.x & 63;
-    const int b = find_block(off, nb, fid);
-    const FBlock &B = blocks[b];
+    const long long sid = tmap[fid];                          // where the state pass looks for this face
+    const FaceRec R = recs[sid];
+    const FBlock &B = blocks[R.b];
     const Consts &C = B.c;
-    const FaceLoc L = face_loc_tiled(B, fid - off[b]);
-    const long long sid = off[b] + face_index(B, L);          // where the state pass looks for this face
-    FaceGeo g = face_geo(B, L.gi, L.gj, L.face);
+    const FaceLoc L{R.face, R.gi - C.G, R.gj - C.G, R.gi, R.gj};
+    FaceGeo g = face_geo_of(R, C);
     uniform_geo(g);
     const Stencil<T> stn = stencil<T, M>(C, B.f, g);
     const d2 n = B.normal[g.own * 2 + L.face];
     const bool wantQ = (L.face == SOUTH) && (L.mj < C.nj);
     const d4 cm = wantQ ? B.macro[(size_t)L.mi * C.nj + L.mj] : mk4(0, 0, 0, 1);
     const double rK = 1.0 / C.K;
-    double a[kNS];
+    double a[kFS];            // kNS sums and one pad, so the reduction folds 16
 #pragma unroll
-    for (int q = 0; q < kNS; q++) a[q] = 0.0;
+    for (int q = 0; q < kFS; q++) a[q] = 0.0;
     face_velocity_loop<T, M, false>(C, stn, n, lane, nullptr, [&](int v, d2 e, double, d2 uv, int delta, const StenVals<T> &sv) __attribute__((always_inline)) {
         P2<T> ff, sg, fl, fr;
         reconstruct_from<T, M>(C, g, delta, sv, ff, sg, &fl, &fr);
@@ -610,28 +668,30 @@ __global__ void __launch_bounds__(64 * FW) k_faceA(const FBlock *__restrict__ bl
             a[14] += 0.5 * (cy * t);
         }
     });
-#pragma unroll
-    for (int q = 0; q < kNS; q++) a[q] = wave_sum(a[q]);
-    if (lane == 0) {
-        double *o = fsum + sid * kFS;
-#pragma unroll
-        for (int q = 0; q < kNS; q++) o[q] = a[q];
-        if (wantQ) B.Q[(size_t)L.mi * C.nj + L.mj] = mk2(a[13], a[14]);
+    // row r of the wavefront gets the totals of sums 4r .. 4r+3; its first
+    // lane stores them (the pad, sum 15, is zero)
+    double t[4];
+    wave_sum16_rows(a, t);
+    if ((lane & 15) == 0) {
+        d2 *o = (d2 *)(fsum + sid * kFS + 4 * (lane >> 4));
+        o[0] = mk2(t[0], t[1]);
+        o[1] = mk2(t[2], t[3]);
+        if (wantQ && lane == 48) B.Q[(size_t)L.mi * C.nj + L.mj] = mk2(t[1], t[2]);
     }
 }
 
 // face state: primary variables, slopes, time slope, time integrals, the
 // equilibrium part of the macro flux and the face heat flux (from the raw
 // moments about the face velocity) -- getPLR, initMacroFlux, calcFaceQ
-__global__ void __launch_bounds__(kT) k_faceS(const FBlock *__restrict__ blocks, const long long *__restrict__ off,
-                                              int nb, long long nfaces, const double *__restrict__ fsum,
+__global__ void __launch_bounds__(kT) k_faceS(const FBlock *__restrict__ blocks, const FaceRec *__restrict__ recs,
+                                              long long nfaces, const double *__restrict__ fsum,
                                               double *__restrict__ fstate, const DtState *__restrict__ ds) {
     const long long fid = (long long)blockIdx.x * kT + threadIdx.x;
     if (fid >= nfaces) return;
-    const int b = find_block(off, nb, fid);
-    const FBlock &B = blocks[b];
+    const FaceRec R = recs[fid];
+    const FBlock &B = blocks[R.b];
     const Consts &C = B.c;
-    const FaceLoc L = face_loc(B, fid - off[b]);
+    const FaceLoc L{R.face, R.gi - C.G, R.gj - C.G, R.gi, R.gj};
     const size_t own = (size_t)L.gi * C.NJ + L.gj;
     const long long step = L.face == SOUTH ? 1 : C.NJ;
     const double *S = fsum + fid * kFS;
@@ -642,14 +702,14 @@ __global__ void __launch_bounds__(kT) k_faceS(const FBlock *__restrict__ blocks,
     const d2 uL = to_local(mk2(ML.s1, ML.s2), nf), uR = to_local(mk2(MR.s1, MR.s2), nf);
     const d4 WL = mk4(ML.s0, uL.x, uL.y, ML.s3), WR = mk4(MR.s0, uR.x, uR.y, MR.s3);
     const d4 prim = to_primary(C, W);
-    const d2 ms = B.mid_side[own * 2 + L.face];
-    const d4 aL = micro_slope(C, prim, (W - WL) / length(ms - B.centre[(long long)own - step]));
-    const d4 aR = micro_slope(C, prim, (WR - W) / length(ms - B.centre[own]));
+    // |midside - centre| of the cells below (upwind choice 1) and above (0) the face
+    const d4 aL = micro_slope(C, prim, (W - WL) / R.dist[1]);
+    const d4 aR = micro_slope(C, prim, (WR - W) / R.dist[0]);
     Moments m;
     velocity_moments(C, prim, m);
     const d4 sw = (-prim.s0) * (moment_au(aL, m.MuL, m.Mv, m.Mxi, 1, 0) + moment_au(aR, m.MuR, m.Mv, m.Mxi, 1, 0));
     const d4 aT = micro_slope(C, prim, sw);
-    const TimeInt mt = time_integrals(relax_time(C, prim), ds[B.cs].dt);
+    const TimeInt mt = time_integrals(relax_time(C, prim), ds[B.cs].dt_flux);
     const d4 g0 = prim.s0 * (mt.m0 * moment_uv(m.Mu, m.Mv, m.Mxi, 1, 0, 0) +
                              mt.m1 * (moment_au(aL, m.MuL, m.Mv, m.Mxi, 2, 0) + moment_au(aR, m.MuR, m.Mv, m.Mxi, 2, 0)) +
                              mt.m2 * moment_au(aT, m.Mu, m.Mv, m.Mxi, 1, 0));
@@ -696,19 +756,19 @@ template <typename T, int M> constexpr int kFaceCWaves = (sizeof(T) == 4 && M >=
 // the FP64 flux stores one iteration late (the FP32 variant would spill more at six waves)
 template <typename T> constexpr bool kDeferStore = sizeof(T) == 8;
 template <typename T, int FW, int M>
-__global__ void __launch_bounds__(64 * FW) __attribute__((amdgpu_waves_per_eu(kFaceCWaves<T, M>))) k_faceC(const FBlock *__restrict__ blocks, const long long *__restrict__ off,
-                                                   int nb, long long nfaces, const double *__restrict__ fstate) {
+__global__ void __launch_bounds__(64 * FW) __attribute__((amdgpu_waves_per_eu(kFaceCWaves<T, M>))) k_faceC(const FBlock *__restrict__ blocks, const FaceRec *__restrict__ recs,
+                                                   long long nfaces, const double *__restrict__ fstate) {
     __shared__ double tab[FW][128];
     __shared__ __attribute__((aligned(16))) char sbuf[FW][kStenLds<T>];
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const long long fid = xcd_remap(blockIdx.x, gridDim.x) * FW + wv;
     if (fid >= nfaces) return;
     const int lane = threadIdx.x & 63;
-    const int b = find_block(off, nb, fid);
-    const FBlock &B = blocks[b];
+    const FaceRec R = recs[fid];
+    const FBlock &B = blocks[R.b];
     const Consts &C = B.c;
-    const FaceLoc L = face_loc(B, fid - off[b]);
-    FaceGeo g = face_geo(B, L.gi, L.gj, L.face);
+    const FaceLoc L{R.face, R.gi - C.G, R.gj - C.G, R.gi, R.gj};
+    FaceGeo g = face_geo_of(R, C);
     uniform_geo(g);
     const Stencil<T> stn = stencil<T, M>(C, B.f, g);
     // the face's folded flux constants (k_faceS)
@@ -790,8 +850,7 @@ __global__ void __launch_bounds__(64 * FW) __attribute__((amdgpu_waves_per_eu(kF
         mf[3] += 0.5 * (uu * X + ux * fma(SP2, hk * f0, th));
     });
     if (last >= 0) stream_store<T>(fout + last, pend);
-#pragma unroll
-    for (int q = 0; q < 4; q++) mf[q] = wave_sum(mf[q]);
+    wave_sum4(mf);
     if (lane == 0) {
         d4 o = mk4(mf[0], mf[1], mf[2], mf[3]) + Lf * mk4(sv[24], sv[25], sv[26], sv[27]);
         const d2 gl = to_global(mk2(o.s1, o.s2), n);
@@ -822,7 +881,7 @@ __global__ void __launch_bounds__(kT) k_wall(const FBlock *__restrict__ blocks, 
     case GSOUTH: gi = ci + C.G; gj = C.G; rot = 1; fid = SOUTH; break;
     default: gi = C.G; gj = ci + C.G; rot = 1; fid = WEST; break;
     }
-    const double dt = ds[B.cs].dt;
+    const double dt = ds[B.cs].dt_flux;
     const FaceGeo g = face_geo(B, gi, gj, fid);
     const d2 n = B.normal[g.own * 2 + fid];
     const double L = B.side_length[g.own * 2 + fid];
@@ -1076,6 +1135,15 @@ __device__ __forceinline__ void add_moments(const Consts &C, int v, T gx, T hx, 
     add_moments<T>(quad_at(C, v), gx, hx, m);
 }
 
+// A cell's four moments summed over the wavefront (all 64 lanes active) and
+// stored: the first lane of row r writes component r.  The update, the ghost
+// fills and the per-call refresh all store through here, so a cell's
+// moments come out bit for bit the same whichever of them wrote them.
+__device__ __forceinline__ void store_moments(d4 *dst, const double (&m)[4], int lane) {
+    const double r = wave_sum4_rows(m);
+    if ((lane & 15) == 0) ((double *)dst)[lane >> 4] = r;
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kT) k_cellmom(const FBlock *__restrict__ blocks, const long long *__restrict__ off,
                                                 int nb, long long nall) {
@@ -1092,9 +1160,7 @@ __global__ void __launch_bounds__(kT) k_cellmom(const FBlock *__restrict__ block
         const P2<T> x = F[v];
         add_moments<T>(C, v, x.x, x.y, m);
     }
-#pragma unroll
-    for (int q = 0; q < 4; q++) m[q] = wave_sum(m[q]);
-    if (lane == 0) B.cmom[cell] = mk4(m[0], m[1], m[2], m[3]);
+    store_moments(B.cmom + cell, m, lane);
 }
 
 // ============================================================ k_update ====
@@ -1119,7 +1185,7 @@ __global__ void __launch_bounds__(kT) k_update(const FBlock *__restrict__ blocks
     const long long cell = (long long)mi * C.nj + mj;
     const int gi = mi + C.G, gj = mj + C.G;
     const size_t c0 = (size_t)gi * C.NJ + gj;
-    const double dt = ds[B.cs].dt;
+    const double dt = ds[B.cs].dt_up;
     // updateMacro (clUGKS.c:1158-1187)
     const d4 pold = B.macro[cell];
     const double A = B.area[c0];
@@ -1214,9 +1280,7 @@ __global__ void __launch_bounds__(kT) k_update(const FBlock *__restrict__ blocks
         add_moments<T>(e, tg, th, mom);
     }
     if (nan_seen) B.flag[0] = ERR_NAN;
-#pragma unroll
-    for (int q = 0; q < 4; q++) mom[q] = wave_sum(mom[q]);
-    if (lane == 0) B.cmom[c0] = mk4(mom[0], mom[1], mom[2], mom[3]);
+    store_moments(B.cmom + c0, mom, lane);
 }
 
 // ============================================================ k_dt =========
@@ -1293,14 +1357,75 @@ __global__ void k_dtfin(DtState *dss, unsigned long long *maxbits, double *max_d
         return;
     }
     const double lim = 1.0 + ds->delta_dt;
-    double d1 = ds->dt;
-    if (d1 != M) d1 = (M > lim * d1) ? lim * d1 : M;
-    double d2v = d1;
-    if (d2v != M) d2v = (M > lim * d2v) ? lim * d2v : M;
+    // updateTimeStep after the end-of-step calcTimeStep (ugks_sim.py:457-459),
+    // from the dt the update used; time advances by the result (:491)
+    const double d1 = limited_dt(ds->dt_up, M, lim);
     ds->time += d1;
     ds->step += 1;
-    if (d2v != d1 && ds->step == 1) ds->overshoot = 1;
-    ds->dt = d2v;
+    ds->dt = d1;
+    // the next step's flux loop (ugks_sim.py:430-446): the flux with d1, dt
+    // re-evaluated; if it moved, the flux again with d2 and dt re-evaluated
+    // once more for the update ("Overshot" if that happens on gdata.step 1)
+    const double d2 = limited_dt(d1, M, lim);
+    if (d2 == d1) {
+        ds->dt_flux = ds->dt_up = d1;
+    } else {
+        ds->dt_flux = d2;
+        ds->dt_up = limited_dt(d2, M, lim);
+        if (ds->step == 1) ds->overshoot = 1;
+    }
+}
+
+// paraBC (clKernel.c:141-218) at the start of a step, for the blocks whose
+// wall values are expressions in t (ugks_sim.py:425-427): every wall cell of
+// block list[blockIdx.x] at its case's time
+__global__ void __launch_bounds__(kT) k_parabc(const FBlock *__restrict__ blocks, const int *__restrict__ list,
+                                               const DtState *__restrict__ ds) {
+    const FBlock &B = blocks[list[blockIdx.x]];
+    const Consts &C = B.c;
+    const int ci = blockIdx.y * kT + threadIdx.x;
+    const int wl = C.ni > C.nj ? C.ni : C.nj;
+    if (ci >= wl) return;
+    const double t = ds[B.cs].time;
+    d4 *wp = (d4 *)B.wall_prop;
+    for (int w = 0; w < 4; ++w) {
+        const int along = (w == GNORTH || w == GSOUTH) ? C.ni : C.nj;
+        if (ci < along) wp[(size_t)ci * 4 + w] = para_wall(C, w, B.para[(size_t)ci * 4 + w], t);
+    }
+}
+
+// edge-exchange corner check (clGhostRoutines.c:491-512): a joined face whose
+// end vertices do not coincide with the partner face's (1e-6) has its ghost row
+// flipped, if its orientation flag asks for it.  Thread 4 b + k checks face k
+// of block b from the vertices as they stand, at every run call.
+__global__ void k_flipidx(FBlock *__restrict__ blocks, int nb) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 4 * nb) return;
+    FBlock &F = blocks[t >> 2];
+    const int k = t & 3;
+    int flip = 0;
+    if ((F.bc[k] == 0 || F.bc[k] == 5) && F.ori[k]) {
+        const FBlock &O = blocks[F.other[k]];
+        const int G = F.c.G, NI = F.c.NI, NJ = F.c.NJ, NIB = O.c.NI, NJB = O.c.NJ;
+        int A[4], Bv[4];
+        switch (k) {
+        case GNORTH: A[0] = G; A[1] = NJ - G; A[2] = NI - G; A[3] = NJ - G; break;
+        case GEAST: A[0] = NI - G; A[1] = G; A[2] = NI - G; A[3] = NJ - G; break;
+        case GSOUTH: A[0] = G; A[1] = G; A[2] = NI - G; A[3] = G; break;
+        default: A[0] = G; A[1] = G; A[2] = G; A[3] = NJ - G; break;
+        }
+        switch (F.oface[k]) {
+        case GNORTH: Bv[0] = G; Bv[1] = NJB - G; Bv[2] = NIB - G; Bv[3] = NJB - G; break;
+        case GEAST: Bv[0] = NIB - G; Bv[1] = G; Bv[2] = NIB - G; Bv[3] = NJB - G; break;
+        case GSOUTH: Bv[0] = G; Bv[1] = G; Bv[2] = NIB - G; Bv[3] = G; break;
+        default: Bv[0] = G; Bv[1] = G; Bv[2] = G; Bv[3] = NJB - G; break;
+        }
+        const d2 a0 = F.xy[(size_t)A[0] * (NJ + 1) + A[1]], a1 = F.xy[(size_t)A[2] * (NJ + 1) + A[3]];
+        const d2 b0 = O.xy[(size_t)Bv[0] * (NJB + 1) + Bv[1]], b1 = O.xy[(size_t)Bv[2] * (NJB + 1) + Bv[3]];
+        auto ne = [](double a, double c) { return !(fabs(a - c) <= 1e-6); };
+        flip = ne(a0.x, b0.x) || ne(a0.y, b0.y) || ne(a1.x, b1.x) || ne(a1.y, b1.y);
+    }
+    F.flipidx[k] = flip;
 }
 
 // ============================================================ k_ghost ======
@@ -1344,11 +1469,7 @@ __global__ void __launch_bounds__(kT) k_ghost(const FBlock *__restrict__ blocks,
         ((P2<T> *)B.f)[dst + v] = x;
         add_moments<T>(C, v, x.x, x.y, m);
     };
-    auto finish = [&]() {
-#pragma unroll
-        for (int q = 0; q < 4; q++) m[q] = wave_sum(m[q]);
-        if (lane == 0) B.cmom[(size_t)gi * C.NJ + gj] = mk4(m[0], m[1], m[2], m[3]);
-    };
+    auto finish = [&]() { store_moments(B.cmom + (size_t)gi * C.NJ + gj, m, lane); };
     if (type == 0 || type == 5) {           // edgeExchange (clGhostRoutines.c:407-538)
         const FBlock &O = blocks[B.other[face]];
         int iB = gi, jB = gj;
@@ -1486,6 +1607,10 @@ struct ugks_step_plan {
     DtState *d_state;                   // [ncases]
     int *d_case_first;                  // [ncases + 1] first block of each case
     double *d_fsum, *d_fstate;          // per interior face: raw sums, face state
+    FaceRec *d_frec;                    // per interior face: block, side, owner, distances (plain order)
+    int *d_udf_blocks;                  // blocks whose wall values are re-evaluated every step
+    int nudf, udf_wl;                   // their count and largest wall length
+    int *d_tmap;                        // moment-pass item -> plain face index (tiled order)
     hipStream_t cap_stream;             // stream the step graph is captured on
     hipGraph_t graph;
     hipGraphExec_t graph_exec;
@@ -1497,11 +1622,11 @@ struct ugks_step_plan {
 namespace {
 template <typename T, int FW, int M> void launch_faceA(ugks_step_plan *p, hipStream_t s) {
     hipLaunchKernelGGL((k_faceA<T, FW, M>), dim3((unsigned)((p->nfaces + FW - 1) / FW)), dim3(64 * FW), 0, s,
-                       p->d_blocks, p->d_face_off, p->nb, p->nfaces, p->d_fsum);
+                       p->d_blocks, (const FaceRec *)p->d_frec, (const int *)p->d_tmap, p->nfaces, p->d_fsum);
 }
 template <typename T, int FW, int M> void launch_faceC(ugks_step_plan *p, hipStream_t s) {
     hipLaunchKernelGGL((k_faceC<T, FW, M>), dim3((unsigned)((p->nfaces + FW - 1) / FW)), dim3(64 * FW), 0, s,
-                       p->d_blocks, p->d_face_off, p->nb, p->nfaces, (const double *)p->d_fstate);
+                       p->d_blocks, (const FaceRec *)p->d_frec, p->nfaces, (const double *)p->d_fstate);
 }
 // the face passes specialised for the plan's one reconstruction method (or generic)
 template <typename T, int FW> void faceA_m(ugks_step_plan *p, hipStream_t s) {
@@ -1531,12 +1656,15 @@ template <typename T> void faceC(ugks_step_plan *p, hipStream_t s, int fw) {
 constexpr int kLaunches = UGKS_STEP_NKERNELS;
 template <typename T, int NPT> int launch_step(ugks_step_plan *p, hipStream_t s, hipEvent_t *ev) {
     const unsigned gt = (unsigned)((p->nfaces + kT - 1) / kT);             // one thread per face
+    if (p->nudf > 0)
+        hipLaunchKernelGGL(k_parabc, dim3((unsigned)p->nudf, (unsigned)((p->udf_wl + kT - 1) / kT)), dim3(kT), 0, s,
+                           p->d_blocks, (const int *)p->d_udf_blocks, (const DtState *)p->d_state);
     if (ev) (void)hipEventRecord(ev[0], s);
     if (p->nfaces > 0)
         faceA<T>(p, s, p->fa_waves);
     if (ev) (void)hipEventRecord(ev[1], s);
     if (p->nfaces > 0)
-        hipLaunchKernelGGL(k_faceS, dim3(gt), dim3(kT), 0, s, p->d_blocks, p->d_face_off, p->nb, p->nfaces,
+        hipLaunchKernelGGL(k_faceS, dim3(gt), dim3(kT), 0, s, p->d_blocks, (const FaceRec *)p->d_frec, p->nfaces,
                            (const double *)p->d_fsum, p->d_fstate, (const DtState *)p->d_state);
     if (ev) (void)hipEventRecord(ev[2], s);
     if (p->nfaces > 0)
@@ -1562,11 +1690,16 @@ template <typename T, int NPT> int launch_step(ugks_step_plan *p, hipStream_t s,
     return ugks_rt::check_launch("ugks_step_run");
 }
 
-// moments of every cell from f as it stands (f may have been changed between runs)
+// moments of every cell from f as it stands (f may have been changed between
+// runs), and the face records from the geometry as it stands
 template <typename T> int refresh_moments(ugks_step_plan *p, hipStream_t s) {
     hipLaunchKernelGGL((k_cellmom<T>), dim3((unsigned)((p->nall + kWavesT - 1) / kWavesT)), dim3(kT), 0, s, p->d_blocks,
                        p->d_all_off, p->nb, p->nall);
-    return ugks_rt::check_launch("ugks_step: cell moments");
+    if (p->nfaces > 0)
+        hipLaunchKernelGGL(k_facerec, dim3((unsigned)((p->nfaces + kT - 1) / kT)), dim3(kT), 0, s, p->d_blocks,
+                           p->d_face_off, p->nb, p->nfaces, p->d_frec, p->d_tmap);
+    hipLaunchKernelGGL(k_flipidx, dim3((unsigned)((4 * p->nb + kT - 1) / kT)), dim3(kT), 0, s, p->d_blocks, p->nb);
+    return ugks_rt::check_launch("ugks_step: cell moments, face records and join corners");
 }
 int refresh(ugks_step_plan *p, hipStream_t s) {
     return p->precision == UGKS_FP64 ? refresh_moments<double>(p, s) : refresh_moments<float>(p, s);
@@ -1588,6 +1721,20 @@ template <typename T> int dispatch_npt(ugks_step_plan *p, hipStream_t s, hipEven
 }  // namespace
 
 namespace {
+// The first flux loop of a step whose dt was set from outside (a new plan, or
+// set_dt): M is the smallest block maximum time step from the last
+// calcTimeStep (0 if unknown: dt is then taken as is).
+__host__ __device__ void flux_loop_dts(DtState &d, double M) {
+    d.dt_flux = d.dt_up = d.dt;
+    if (d.force_time || !(M > 0.0)) return;
+    const double lim = 1.0 + d.delta_dt;
+    const double d1 = limited_dt(d.dt, M, lim);
+    if (d1 == d.dt) return;
+    d.dt_flux = d1;
+    d.dt_up = limited_dt(d1, M, lim);
+    if (d.step == 1) d.overshoot = 1;
+}
+
 template <typename T> int autotune(ugks_step_plan *p, hipStream_t s, int reps, int32_t *choice) {
     // The face passes only read f and the step state and rewrite per-step scratch
     // (face sums, face states, fluxes, Q from the same f), so timing them
@@ -1605,7 +1752,7 @@ template <typename T> int autotune(ugks_step_plan *p, hipStream_t s, int reps, i
             // kC reads kA's results through kS: run the chain once, then time the kernel alone
             faceA<T>(p, s, which == 0 ? fw : p->fa_waves);
             hipLaunchKernelGGL(k_faceS, dim3((unsigned)((p->nfaces + kT - 1) / kT)), dim3(kT), 0, s, p->d_blocks,
-                               p->d_face_off, p->nb, p->nfaces, (const double *)p->d_fsum, p->d_fstate,
+                               (const FaceRec *)p->d_frec, p->nfaces, (const double *)p->d_fsum, p->d_fstate,
                                (const DtState *)p->d_state);
             (void)hipEventRecord(e0, s);
             for (int r = 0; r < reps; r++) {
@@ -1630,6 +1777,58 @@ template <typename T> int autotune(ugks_step_plan *p, hipStream_t s, int reps, i
     if (p->graph_exec) { (void)hipGraphExecDestroy(p->graph_exec); p->graph_exec = nullptr; }
     if (p->graph) { (void)hipGraphDestroy(p->graph); p->graph = nullptr; }
     return 0;
+}
+}  // namespace
+
+namespace {
+// One block's device record from its descriptor, buffers and boundary
+// conditions, with the checks every plan makes; wallf[k] tells which faces
+// are walls (diffuse or adsorbing: their faces leave the interior face list).
+int fill_block(FBlock &F, const ugks_block *blk, const ugks_block_buffers &u, const ugks_bc_desc *bc4, int cs,
+               int nblocks, const int32_t *case_of_block, int precision, bool (&wallf)[4]) {
+    const Consts &C = blk->c;
+    memset(&F, 0, sizeof F);
+    F.c = C;
+    F.cs = cs;
+    F.f = u.f; F.fluxS = u.flux_f_S; F.fluxW = u.flux_f_W; F.wdist = u.wall_dist;
+    F.fmS = (d4 *)u.flux_macro_S; F.fmW = (d4 *)u.flux_macro_W;
+    F.macro = (d4 *)u.macro; F.macro_new = (d4 *)u.macro_new; F.residual = (d4 *)u.residual;
+    F.wall_cover = (d4 *)u.wall_cover; F.Q = (d2 *)u.Q;
+    F.xy = (const d2 *)u.xy; F.centre = (const d2 *)u.centre; F.mid_side = (const d2 *)u.mid_side;
+    F.normal = (const d2 *)u.normal; F.area = (const double *)u.area; F.side_length = (const double *)u.side_length;
+    F.wall_prop = (const d4 *)u.wall_prop; F.wall_fluxes = (double *)u.wall_fluxes;
+    F.time_step = (double *)u.time_step; F.flag = (double *)u.flag;
+    F.para = (const d4 *)u.para;
+    if (C.time_udf && !u.para) return ugks_rt::fail(-1, "step plan: time-dependent wall values need para");
+    const void *need[] = {u.f, u.flux_f_S, u.flux_f_W, u.flux_macro_S, u.flux_macro_W, u.macro, u.macro_new,
+                          u.residual, u.Q, u.xy, u.centre, u.mid_side, u.normal, u.area, u.side_length, u.wall_prop,
+                          u.wall_cover, u.wall_fluxes, u.time_step, u.flag};
+    for (const void *q : need)
+        if (!q) return ugks_rt::fail(-1, "step plan: a required buffer is null");
+    for (int k = 0; k < 4; k++) {
+        const ugks_bc_desc &d = bc4[k];
+        F.bc[k] = d.type; F.other[k] = d.other_block; F.oface[k] = d.other_face; F.ori[k] = d.orientation;
+        F.flip[k] = d.flip; F.reflect[k] = d.reflect_type;
+        if (d.type < 0 || d.type > 9) return ugks_rt::fail(-1, "step plan: bad BC type");
+        if ((d.type == 0 || d.type == 5) && (d.other_block < 0 || d.other_block >= nblocks))
+            return ugks_rt::fail(-1, "step plan: exchange partner out of range");
+        if ((d.type == 0 || d.type == 5) && case_of_block && case_of_block[d.other_block] != F.cs)
+            return ugks_rt::fail(-1, "step plan: exchange partner belongs to another case");
+        if (d.type == 8 && !C.cads[k] && C.n_tri[k] <= 0)
+            return ugks_rt::fail(-1, "step plan: adsorbing wall without an isotherm");
+        wallf[k] = (d.type == 4 || d.type == 8);
+    }
+    F.ob_S = wallf[GSOUTH] ? 1 : 0;
+    const int ot_S = wallf[GNORTH] ? 1 : 0;
+    F.ob_W = wallf[GWEST] ? 1 : 0;
+    const int ot_W = wallf[GEAST] ? 1 : 0;
+    F.nSj = C.nj + 1 - F.ob_S - ot_S;
+    F.nS = (int)((long long)C.ni * F.nSj);
+    F.nWi = C.ni + 1 - F.ob_W - ot_W;
+    F.wl = C.ni > C.nj ? C.ni : C.nj;
+    if ((long long)F.nS + (long long)F.nWi * C.nj >= (1LL << 31) - (long long)kTileI * kTileJ * 2)
+        return ugks_rt::fail(-2, "step plan: too many faces in one block");
+    return ugks_step_block_fits(C.NJ, C.NV, C.method, precision);
 }
 }  // namespace
 
@@ -1669,58 +1868,20 @@ int ugks_step_plan_create_batch(ugks_block *const *blocks, const ugks_block_buff
         tile_off(nblocks + 1, 0);
     std::vector<WallItem> walls;
     std::vector<GhostItem> ghosts;
+    std::vector<int> udf_blocks;          // blocks whose wall values depend on time
     long long nw = 0, ng = 0;
     p->host_blocks.resize(nblocks);
     for (int b = 0; b < nblocks; b++) {
         if (!blocks[b]) { delete p; return ugks_rt::fail(-1, "step plan: null block"); }
         const Consts &C = blocks[b]->c;
-        const ugks_block_buffers &u = bufs[b];
         FBlock &F = p->host_blocks[b];
-        memset(&F, 0, sizeof F);
-        F.c = C;
-        F.cs = case_of_block ? case_of_block[b] : 0;
-        F.f = u.f; F.fluxS = u.flux_f_S; F.fluxW = u.flux_f_W; F.wdist = u.wall_dist;
-        F.fmS = (d4 *)u.flux_macro_S; F.fmW = (d4 *)u.flux_macro_W;
-        F.macro = (d4 *)u.macro; F.macro_new = (d4 *)u.macro_new; F.residual = (d4 *)u.residual;
-        F.wall_cover = (d4 *)u.wall_cover; F.Q = (d2 *)u.Q;
-        F.xy = (const d2 *)u.xy; F.centre = (const d2 *)u.centre; F.mid_side = (const d2 *)u.mid_side;
-        F.normal = (const d2 *)u.normal; F.area = (const double *)u.area; F.side_length = (const double *)u.side_length;
-        F.wall_prop = (const d4 *)u.wall_prop; F.wall_fluxes = (double *)u.wall_fluxes;
-        F.time_step = (double *)u.time_step; F.flag = (double *)u.flag;
-        void *need[] = {u.f, u.flux_f_S, u.flux_f_W, u.flux_macro_S, u.flux_macro_W, u.macro, u.macro_new, u.residual,
-                        u.Q, (void *)u.xy, (void *)u.centre, (void *)u.mid_side, (void *)u.normal, (void *)u.area,
-                        (void *)u.side_length, u.wall_prop, u.wall_cover, u.wall_fluxes, u.time_step, u.flag};
-        for (void *q : need)
-            if (!q) { delete p; return ugks_rt::fail(-1, "step plan: a required buffer is null"); }
         bool wallf[4];
-        for (int k = 0; k < 4; k++) {
-            const ugks_bc_desc &d = bcs[4 * b + k];
-            F.bc[k] = d.type; F.other[k] = d.other_block; F.oface[k] = d.other_face; F.ori[k] = d.orientation;
-            F.flip[k] = d.flip; F.reflect[k] = d.reflect_type;
-            if (d.type < 0 || d.type > 9) { delete p; return ugks_rt::fail(-1, "step plan: bad BC type"); }
-            if ((d.type == 0 || d.type == 5) && (d.other_block < 0 || d.other_block >= nblocks)) {
-                delete p; return ugks_rt::fail(-1, "step plan: exchange partner out of range");
-            }
-            if ((d.type == 0 || d.type == 5) && case_of_block && case_of_block[d.other_block] != F.cs) {
-                delete p; return ugks_rt::fail(-1, "step plan: exchange partner belongs to another case");
-            }
-            if (d.type == 8 && !C.cads[k] && C.n_tri[k] <= 0) {
-                delete p; return ugks_rt::fail(-1, "step plan: adsorbing wall without an isotherm");
-            }
-            wallf[k] = (d.type == 4 || d.type == 8);
+        if (int rc = fill_block(F, blocks[b], bufs[b], bcs + 4 * b, case_of_block ? case_of_block[b] : 0, nblocks,
+                                case_of_block, precision, wallf)) {
+            delete p;
+            return rc;
         }
-        F.ob_S = wallf[GSOUTH] ? 1 : 0;
-        int ot_S = wallf[GNORTH] ? 1 : 0;
-        F.ob_W = wallf[GWEST] ? 1 : 0;
-        int ot_W = wallf[GEAST] ? 1 : 0;
-        F.nSj = C.nj + 1 - F.ob_S - ot_S;
-        F.nS = (int)((long long)C.ni * F.nSj);
-        F.nWi = C.ni + 1 - F.ob_W - ot_W;
-        F.wl = C.ni > C.nj ? C.ni : C.nj;
-        if ((long long)F.nS + (long long)F.nWi * C.nj >= (1LL << 31) - (long long)kTileI * kTileJ * 2) {
-            delete p; return ugks_rt::fail(-2, "step plan: too many faces in one block");
-        }
-        if (int rc = ugks_step_block_fits(C.NJ, C.NV, C.method, precision)) { delete p; return rc; }
+        if (C.time_udf) udf_blocks.push_back(b);
         face_off[b + 1] = face_off[b] + F.nS + (long long)F.nWi * C.nj;
         cell_off[b + 1] = cell_off[b] + (long long)C.ni * C.nj;
         all_off[b + 1] = all_off[b] + (long long)C.NI * C.NJ;
@@ -1733,39 +1894,15 @@ int ugks_step_plan_create_batch(ugks_block *const *blocks, const ugks_block_buff
         }
         if (C.NV > maxnv) maxnv = C.NV;
     }
-    // edge-exchange corner check (clGhostRoutines.c:491-512), once per plan
+    // joined blocks must agree in velocities and ghost width (the corner check itself runs on the device)
     for (int b = 0; b < nblocks; b++) {
-        FBlock &F = p->host_blocks[b];
+        const FBlock &F = p->host_blocks[b];
         for (int k = 0; k < 4; k++) {
             if (!(F.bc[k] == 0 || F.bc[k] == 5)) continue;
             const FBlock &O = p->host_blocks[F.other[k]];
             if (O.c.NV != F.c.NV || O.c.G != F.c.G) {
                 delete p; return ugks_rt::fail(-1, "step plan: joined blocks differ in NV or ghost width");
             }
-            if (!F.ori[k]) continue;
-            const int G = F.c.G, NI = F.c.NI, NJ = F.c.NJ, NIB = O.c.NI, NJB = O.c.NJ;
-            int A[4], Bv[4];
-            switch (k) {
-            case GNORTH: A[0] = G; A[1] = NJ - G; A[2] = NI - G; A[3] = NJ - G; break;
-            case GEAST: A[0] = NI - G; A[1] = G; A[2] = NI - G; A[3] = NJ - G; break;
-            case GSOUTH: A[0] = G; A[1] = G; A[2] = NI - G; A[3] = G; break;
-            default: A[0] = G; A[1] = G; A[2] = G; A[3] = NJ - G; break;
-            }
-            switch (F.oface[k]) {
-            case GNORTH: Bv[0] = G; Bv[1] = NJB - G; Bv[2] = NIB - G; Bv[3] = NJB - G; break;
-            case GEAST: Bv[0] = NIB - G; Bv[1] = G; Bv[2] = NIB - G; Bv[3] = NJB - G; break;
-            case GSOUTH: Bv[0] = G; Bv[1] = G; Bv[2] = NIB - G; Bv[3] = G; break;
-            default: Bv[0] = G; Bv[1] = G; Bv[2] = G; Bv[3] = NJB - G; break;
-            }
-            d2 va[2], vb[2];
-            int e = 0;
-            e |= ugks_rt::copy_small(&va[0], F.xy + (size_t)A[0] * (NJ + 1) + A[1], sizeof(d2), 0);
-            e |= ugks_rt::copy_small(&va[1], F.xy + (size_t)A[2] * (NJ + 1) + A[3], sizeof(d2), 0);
-            e |= ugks_rt::copy_small(&vb[0], O.xy + (size_t)Bv[0] * (NJB + 1) + Bv[1], sizeof(d2), 0);
-            e |= ugks_rt::copy_small(&vb[1], O.xy + (size_t)Bv[2] * (NJB + 1) + Bv[3], sizeof(d2), 0);
-            if (e) { delete p; return ugks_rt::fail(-1, "step plan: vertex read failed"); }
-            auto ne = [](double a, double c) { return !(fabs(a - c) <= 1e-6); };
-            F.flipidx[k] = ne(va[0].x, vb[0].x) || ne(va[0].y, vb[0].y) || ne(va[1].x, vb[1].x) || ne(va[1].y, vb[1].y);
         }
     }
     int npt = (maxnv + kT - 1) / kT;
@@ -1777,6 +1914,7 @@ int ugks_step_plan_create_batch(ugks_block *const *blocks, const ugks_block_buff
             delete p; return ugks_rt::fail(-2, "step plan: more than 64 velocities per axis");
         }
     p->nfaces = face_off[nblocks];
+    if (p->nfaces >= (1LL << 31)) { delete p; return ugks_rt::fail(-2, "step plan: 2^31 or more interior faces"); }
     p->ncells = cell_off[nblocks];
     p->nall = all_off[nblocks];
     p->ntiles = tile_off[nblocks];
@@ -1801,11 +1939,19 @@ int ugks_step_plan_create_batch(ugks_block *const *blocks, const ugks_block_buff
     (void)alloc((void **)&p->d_max_dt, sizeof(double) * nblocks);
     (void)alloc((void **)&p->d_fsum, sizeof(double) * kFS * (size_t)p->nfaces);
     (void)alloc((void **)&p->d_fstate, sizeof(double) * kFT * (size_t)p->nfaces);
+    (void)alloc((void **)&p->d_frec, sizeof(FaceRec) * (size_t)p->nfaces);
+    (void)alloc((void **)&p->d_tmap, sizeof(int) * (size_t)p->nfaces);
+    (void)alloc((void **)&p->d_udf_blocks, sizeof(int) * (udf_blocks.size() ? udf_blocks.size() : 1));
+    p->nudf = (int)udf_blocks.size();
+    p->udf_wl = 0;
+    for (int b : udf_blocks) p->udf_wl = std::max(p->udf_wl, std::max(p->host_blocks[b].c.ni, p->host_blocks[b].c.nj));
     if (e != hipSuccess) { ugks_step_plan_destroy(p); return ugks_rt::hip_fail(e, "step plan: hipMalloc"); }
     std::vector<DtState> st(ncases);
-    for (int c = 0; c < ncases; c++)
-        st[c] = DtState{dtd[c].dt, dtd[c].time, dtd[c].CFL, dtd[c].delta_dt, dtd[c].suggest_dt, dtd[c].Kn_eff,
-                        dtd[c].step, 0, dtd[c].force_time, 0};
+    for (int c = 0; c < ncases; c++) {
+        st[c] = DtState{dtd[c].dt, dtd[c].dt, dtd[c].dt, dtd[c].time, dtd[c].CFL, dtd[c].delta_dt, dtd[c].suggest_dt,
+                        dtd[c].Kn_eff, dtd[c].step, 0, dtd[c].force_time, 0};
+        flux_loop_dts(st[c], dtd[c].max_dt);
+    }
     std::vector<unsigned long long> zeros(nblocks, 0ull);
     if (e == hipSuccess)
         for (int b = 0; b < nblocks; b++) p->host_blocks[b].cmom = p->d_cmom + all_off[b];
@@ -1814,6 +1960,8 @@ int ugks_step_plan_create_batch(ugks_block *const *blocks, const ugks_block_buff
     if (e == hipSuccess) e = hipMemcpy(p->d_cell_off, cell_off.data(), sizeof(long long) * (nblocks + 1), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(p->d_all_off, all_off.data(), sizeof(long long) * (nblocks + 1), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(p->d_tile_off, tile_off.data(), sizeof(long long) * (nblocks + 1), hipMemcpyHostToDevice);
+    if (e == hipSuccess && p->nudf)
+        e = hipMemcpy(p->d_udf_blocks, udf_blocks.data(), sizeof(int) * udf_blocks.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess && walls.size())
         e = hipMemcpy(p->d_walls, walls.data(), sizeof(WallItem) * walls.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(p->d_ghosts, ghosts.data(), sizeof(GhostItem) * ghosts.size(), hipMemcpyHostToDevice);
@@ -1821,7 +1969,14 @@ int ugks_step_plan_create_batch(ugks_block *const *blocks, const ugks_block_buff
     if (e == hipSuccess)
         e = hipMemcpy(p->d_case_first, case_first.data(), sizeof(int) * (ncases + 1), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(p->d_maxbits, zeros.data(), sizeof(unsigned long long) * nblocks, hipMemcpyHostToDevice);
+    std::vector<double> md0(nblocks);
+    for (int b = 0; b < nblocks; b++) md0[b] = dtd[case_of_block ? case_of_block[b] : 0].max_dt;
+    if (e == hipSuccess) e = hipMemcpy(p->d_max_dt, md0.data(), sizeof(double) * nblocks, hipMemcpyHostToDevice);
     if (e != hipSuccess) { ugks_step_plan_destroy(p); return ugks_rt::hip_fail(e, "step plan: upload"); }
+    // face records and cell moments before anything (the autotuner included) runs a face pass
+    if (int rc = refresh(p, nullptr)) { ugks_step_plan_destroy(p); return rc; }
+    e = hipStreamSynchronize(nullptr);
+    if (e != hipSuccess) { ugks_step_plan_destroy(p); return ugks_rt::hip_fail(e, "step plan: face records"); }
     *out = p;
     return 0;
 }
@@ -1837,7 +1992,8 @@ int ugks_step_plan_destroy(ugks_step_plan *p) {
     if (p->graph) (void)hipGraphDestroy(p->graph);
     if (p->cap_stream) (void)hipStreamDestroy(p->cap_stream);
     void *ptrs[] = {p->d_blocks, p->d_face_off, p->d_cell_off, p->d_all_off, p->d_tile_off, p->d_cmom, p->d_walls, p->d_ghosts,
-                    p->d_state, p->d_case_first, p->d_maxbits, p->d_max_dt, p->d_fsum, p->d_fstate};
+                    p->d_state, p->d_case_first, p->d_maxbits, p->d_max_dt, p->d_fsum, p->d_fstate, p->d_frec,
+                    p->d_tmap, p->d_udf_blocks};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     delete p;
@@ -1882,16 +2038,25 @@ int ugks_step_run_timed(ugks_step_plan *p, void *stream, int32_t nsteps, float *
 int ugks_step_states(ugks_step_plan *p, void *stream, ugks_dt_desc *out, double *max_dt) {
     if (!p || !out) return ugks_rt::fail(-1, "step state: bad arguments");
     std::vector<DtState> st(p->ncases);
+    std::vector<double> md(p->nb);
+    std::vector<int> first(p->ncases + 1);
     hipError_t e = hipMemcpyAsync(st.data(), p->d_state, sizeof(DtState) * p->ncases, hipMemcpyDeviceToHost,
                                   (hipStream_t)stream);
-    if (e == hipSuccess && max_dt)
-        e = hipMemcpyAsync(max_dt, p->d_max_dt, sizeof(double) * p->nb, hipMemcpyDeviceToHost, (hipStream_t)stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(md.data(), p->d_max_dt, sizeof(double) * p->nb, hipMemcpyDeviceToHost, (hipStream_t)stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(first.data(), p->d_case_first, sizeof(int) * (p->ncases + 1), hipMemcpyDeviceToHost,
+                           (hipStream_t)stream);
     if (e == hipSuccess) e = hipStreamSynchronize((hipStream_t)stream);
     if (e != hipSuccess) return ugks_rt::hip_fail(e, "step state");
+    if (max_dt) memcpy(max_dt, md.data(), sizeof(double) * p->nb);
     for (int c = 0; c < p->ncases; c++) {
         out[c].dt = st[c].dt; out[c].time = st[c].time; out[c].CFL = st[c].CFL; out[c].delta_dt = st[c].delta_dt;
         out[c].suggest_dt = st[c].suggest_dt; out[c].Kn_eff = st[c].Kn; out[c].step = st[c].step;
         out[c].overshoot = st[c].overshoot; out[c].force_time = st[c].force_time; out[c].pad = 0;
+        double M = 0.0;
+        for (int b = first[c]; b < first[c + 1]; b++) M = (b == first[c] || md[b] < M) ? md[b] : M;
+        out[c].max_dt = M;
     }
     return 0;
 }
@@ -1909,9 +2074,26 @@ int ugks_step_state(ugks_step_plan *p, void *stream, ugks_dt_desc *out, double *
 
 int ugks_step_set_dt_case(ugks_step_plan *p, void *stream, int32_t case_index, double dt) {
     if (!p || case_index < 0 || case_index >= p->ncases) return ugks_rt::fail(-1, "set dt: bad arguments");
-    hipError_t e = hipMemcpyAsync(&p->d_state[case_index].dt, &dt, sizeof(double), hipMemcpyHostToDevice,
-                                  (hipStream_t)stream);
-    if (e == hipSuccess) e = hipStreamSynchronize((hipStream_t)stream);
+    // the next step re-evaluates dt against the blocks' last maximum time steps
+    DtState d;
+    std::vector<int> first(p->ncases + 1);
+    hipStream_t s = (hipStream_t)stream;
+    hipError_t e = hipMemcpyAsync(&d, p->d_state + case_index, sizeof d, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(first.data(), p->d_case_first, sizeof(int) * (p->ncases + 1),
+                                            hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return ugks_rt::hip_fail(e, "set dt");
+    const int b0 = first[case_index], nb = first[case_index + 1] - b0;
+    std::vector<double> md(nb > 0 ? nb : 1, 0.0);
+    if (nb > 0) e = hipMemcpyAsync(md.data(), p->d_max_dt + b0, sizeof(double) * nb, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return ugks_rt::hip_fail(e, "set dt");
+    double M = 0.0;
+    for (int k = 0; k < nb; k++) M = (k == 0 || md[k] < M) ? md[k] : M;
+    d.dt = d.dt_flux = d.dt_up = dt;
+    flux_loop_dts(d, M);
+    e = hipMemcpyAsync(p->d_state + case_index, &d, sizeof d, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
     return e == hipSuccess ? 0 : ugks_rt::hip_fail(e, "set dt");
 }
 
@@ -1921,7 +2103,8 @@ int ugks_step_force_dt_case(ugks_step_plan *p, void *stream, int32_t case_index,
     if (!p || case_index < 0 || case_index >= p->ncases) return ugks_rt::fail(-1, "force dt: bad arguments");
     DtState *d = p->d_state + case_index;
     const int one = 1;
-    hipError_t e = hipMemcpyAsync(&d->dt, &dt, sizeof(double), hipMemcpyHostToDevice, (hipStream_t)stream);
+    const double dts[3] = {dt, dt, dt};     // dt, dt_flux, dt_up: held, nothing re-evaluates them
+    hipError_t e = hipMemcpyAsync(&d->dt, dts, sizeof dts, hipMemcpyHostToDevice, (hipStream_t)stream);
     if (e == hipSuccess) e = hipMemcpyAsync(&d->force_time, &one, sizeof(int), hipMemcpyHostToDevice, (hipStream_t)stream);
     if (e == hipSuccess) e = hipStreamSynchronize((hipStream_t)stream);
     return e == hipSuccess ? 0 : ugks_rt::hip_fail(e, "force dt");
@@ -1997,3 +2180,4 @@ int ugks_f32_to_f64(void *stream, const void *src, void *dst, int64_t n) {
 }
 
 }  // extern "C"
+

@@ -63,8 +63,9 @@ __global__ void k_cellGeom(Consts C, const d2 *xy, double *area, d2 *centre, d2 
     side_length[sidx(C, i, j, WEST)] = sqrt((Cc.x - D.x) * (Cc.x - D.x) + (Cc.y - D.y) * (Cc.y - D.y));
 }
 
-// paraBC with constant wall values (clKernel.c:141-218)
-__global__ void k_paraBC(Consts C, d4 *wall_prop) {
+// paraBC (clKernel.c:141-218): constant wall values, or expressions in the
+// wall cell's (s, x, y) = para_def and the time t
+__global__ void k_paraBC(Consts C, const d4 *para_def, d4 *wall_prop, double t) {
     long long ci = thread_id();
     int wl = C.ni > C.nj ? C.ni : C.nj;
     if (ci >= wl) return;
@@ -72,12 +73,8 @@ __global__ void k_paraBC(Consts C, d4 *wall_prop) {
     for (int w = 0; w < 4; ++w) {
         int along = (w == GNORTH || w == GSOUTH) ? C.ni : C.nj;
         if (ci >= along) continue;
-        d4 p;
-        p.s0 = C.wall_D[w];
-        p.s1 = C.wall_U[w];
-        p.s2 = C.wall_V[w];
-        p.s3 = (C.wall_P[w] != -1) ? C.wall_P[w] / C.wall_D[w] : C.wall_T[w];
-        wall_prop[(size_t)ci * 4 + w] = p;
+        const d4 pa = C.any_udf ? para_def[(size_t)ci * 4 + w] : d4{0.0, 0.0, 0.0, 0.0};
+        wall_prop[(size_t)ci * 4 + w] = para_wall(C, w, pa, t);
     }
 }
 
@@ -825,11 +822,12 @@ int ugks_cellGeom(ugks_block *blk, void *stream, const void *xy, void *area, voi
 }
 
 int ugks_paraBC(ugks_block *blk, void *stream, const void *para_def, void *wall_prop, double t) {
-    (void)para_def; (void)t;   // only constant wall values are supported (no UDF text)
     UGKS_BLK_CHECK(blk);
     const Consts &C = blk->c;
+    if (C.any_udf && !para_def) return ugks_rt::fail(-1, "paraBC: wall-value expressions need para_def");
     long long n = C.ni > C.nj ? C.ni : C.nj;
-    hipLaunchKernelGGL(k_paraBC, dim3(blocks_for_threads(n)), dim3(kBlock), 0, S(stream), C, (d4 *)wall_prop);
+    hipLaunchKernelGGL(k_paraBC, dim3(blocks_for_threads(n)), dim3(kBlock), 0, S(stream), C, (const d4 *)para_def,
+                       (d4 *)wall_prop, t);
     return ugks_rt::check_launch("paraBC");
 }
 

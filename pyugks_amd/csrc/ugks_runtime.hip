@@ -90,8 +90,40 @@ int ugks_block_create(const ugks_block_desc *d, ugks_block **out) {
         iso_doubles += 3 * (size_t)ni_;
         iso_ints += 6 * (size_t)nt;
     }
+    // wall-value programs: opcodes known, operands in range, the stack within 16 entries and
+    // ending with one value, so the device interpreter never leaves its arrays
+    bool any_udf = false;
+    for (int k = 0; k < 16; k++) {
+        const int32_t st = d->udf_start[k], n = d->udf_len[k];
+        if (n == 0) continue;
+        any_udf = true;
+        if (!d->udf_code || n < 0 || st < 0 || (long long)st + n > d->udf_ncode)
+            return fail(-1, "wall-value program out of its code table");
+        int sp = 0;
+        for (int i = 0; i < n; i++) {
+            const int32_t op = d->udf_code[st + i];
+            if (op == 1) {
+                if (i + 1 >= n || !d->udf_const || d->udf_code[st + i + 1] < 0 || d->udf_code[st + i + 1] >= d->udf_nconst)
+                    return fail(-1, "wall-value program: constant index out of range");
+                i++; sp++;
+            } else if (op >= 2 && op <= 5) {
+                sp++;
+            } else if ((op >= 10 && op <= 13) || op == 15 || (op >= 40 && op <= 43)) {
+                if (sp < 2) return fail(-1, "wall-value program: stack underflow");
+                sp--;
+            } else if (op == 14 || (op >= 20 && op <= 35)) {
+                if (sp < 1) return fail(-1, "wall-value program: stack underflow");
+            } else {
+                return fail(-1, "wall-value program: unknown opcode");
+            }
+            if (sp > 16) return fail(-1, "wall-value program: deeper than 16 stack entries");
+        }
+        if (sp != 1) return fail(-1, "wall-value program must leave exactly one value");
+    }
+    const size_t udf_ints = any_udf ? (size_t)d->udf_ncode : 0, udf_doubles = any_udf ? (size_t)d->udf_nconst : 0;
     const size_t bytes = nv * (2 * sizeof(double) + sizeof(double) + 3 * sizeof(int64_t)) + (nut + nvt) * sizeof(double) +
-                         iso_doubles * sizeof(double) + iso_ints * sizeof(int32_t);
+                         iso_doubles * sizeof(double) + udf_doubles * sizeof(double) + iso_ints * sizeof(int32_t) +
+                         udf_ints * sizeof(int32_t);
     ugks_block *b = new ugks_block();
     hipError_t e = hipGetDevice(&b->device);
     if (e != hipSuccess) { delete b; return hip_fail(e, "hipGetDevice"); }
@@ -122,11 +154,13 @@ int ugks_block_create(const ugks_block_desc *d, ugks_block **out) {
     p += nvt * sizeof(double);
     for (int k = 0; k < 4; k++)
         if (d->n_tri[k] > 0) { memcpy(p, d->iso[k], 3 * (size_t)d->n_iso[k] * sizeof(double)); p += 3 * (size_t)d->n_iso[k] * sizeof(double); }
+    if (udf_doubles) { memcpy(p, d->udf_const, udf_doubles * sizeof(double)); p += udf_doubles * sizeof(double); }
     for (int k = 0; k < 4; k++)
         if (d->n_tri[k] > 0) {
             memcpy(p, d->iso_tri[k], 3 * (size_t)d->n_tri[k] * sizeof(int32_t)); p += 3 * (size_t)d->n_tri[k] * sizeof(int32_t);
             memcpy(p, d->iso_nbr[k], 3 * (size_t)d->n_tri[k] * sizeof(int32_t)); p += 3 * (size_t)d->n_tri[k] * sizeof(int32_t);
         }
+    if (udf_ints) { memcpy(p, d->udf_code, udf_ints * sizeof(int32_t)); p += udf_ints * sizeof(int32_t); }
     e = hipMemcpy(b->dev_tables, host, bytes, hipMemcpyHostToDevice);
     free(host);
     if (e != hipSuccess) { (void)hipFree(b->dev_tables); delete b; return hip_fail(e, "hipMemcpy(tables)"); }
@@ -158,11 +192,26 @@ int ugks_block_create(const ugks_block_desc *d, ugks_block **out) {
         c.iso[k] = nullptr; c.iso_tri[k] = c.iso_nbr[k] = nullptr;
         if (d->n_tri[k] > 0) { c.iso[k] = (const double *)dp; dp += 3 * (size_t)d->n_iso[k] * sizeof(double); }
     }
+    c.udf_const = udf_doubles ? (const double *)dp : nullptr;
+    dp += udf_doubles * sizeof(double);
     for (int k = 0; k < 4; k++)
         if (d->n_tri[k] > 0) {
             c.iso_tri[k] = (const int *)dp; dp += 3 * (size_t)d->n_tri[k] * sizeof(int32_t);
             c.iso_nbr[k] = (const int *)dp; dp += 3 * (size_t)d->n_tri[k] * sizeof(int32_t);
         }
+    c.udf_code = udf_ints ? (const int *)dp : nullptr;
+    c.any_udf = any_udf ? 1 : 0;
+    c.time_udf = 0;
+    for (int k = 0; k < 16; k++) {
+        c.udf_start[k] = d->udf_start[k];
+        c.udf_len[k] = any_udf ? d->udf_len[k] : 0;
+        // a program that reads t makes the block's wall values change every step
+        for (int i = 0; c.udf_len[k] && i < d->udf_len[k]; i++) {
+            const int32_t op = d->udf_code[d->udf_start[k] + i];
+            if (op == 1) { i++; continue; }
+            if (op == 5) c.time_udf = 1;
+        }
+    }
     *out = b;
     return 0;
 }

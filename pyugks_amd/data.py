@@ -326,61 +326,10 @@ class UGKSData:
         return {"vanLeer": 0, "WENO5": 1, "minmod": 3}[self.flux_method]
 
     def velocity_tables(self):
-        """QUAD/WEIGHT as the kernels see them ('%0.15e', ugks_CL.py:318-346)."""
-        q = np.array([[bake_e15(a), bake_e15(b)] for a, b in self.quad], dtype=np.float64)
-        w = np.array([bake_e15(a) for a in self.weight], dtype=np.float64)
-        nv = self.Nv
-        zeros = np.zeros(nv, dtype=np.int64)
-        mns = self.mirror_NS if self.mirror_NS is not None else zeros
-        mew = self.mirror_EW if self.mirror_EW is not None else zeros
-        md = self.mirror_D if self.mirror_D is not None else zeros
-        return (np.ascontiguousarray(q), np.ascontiguousarray(w), np.ascontiguousarray(mns, dtype=np.int64),
-                np.ascontiguousarray(mew, dtype=np.int64), np.ascontiguousarray(md, dtype=np.int64))
+        return velocity_tables(self)
 
     def block_constants(self, b: BlockDef) -> dict:
-        """The #define set genHeader emits for one block (ugks_CL.py:58-310)."""
-        G = self.ghost
-        c = dict(ni=b.ni, nj=b.nj, NI=b.ni + 2 * G, NJ=b.nj + 2 * G, NV=self.Nv, ghost=G,
-                 flux_method=self.flux_method_code, relax_type=int(self.relax_type),
-                 Pr=bake_str(self.Pr), Kn_eff=bake_e15(self.Kn_eff), gam=bake_str(self.gamma),
-                 B=bake_str(self.b), K=bake_str(self.K), omega=bake_str(self.omega),
-                 CFL=bake_e15(self.CFL), umax=bake_str(self.umax), vmax=bake_str(self.vmax))
-        wD, wU, wV, wT, wP = [], [], [], [], []
-        bn, bt, gf, ap, dyn, cad, ads = [], [], [], [], [], [], []
-        an, at = [], []
-        iso, iso_tri, iso_nbr = [None] * 4, [None] * 4, [None] * 4
-        for bcx in b.bc_list:
-            if bcx.type_of_BC in (bcd.ADSORBING, bcd.DIFFUSE, bcd.CONSTANT, bcd.INFLOW, bcd.OUTFLOW):
-                wD.append(bake_str(bcx.D))
-                wU.append(bake_str(bcx.U))
-                wV.append(bake_str(bcx.V))
-                with np.errstate(divide="ignore"):       # '1.0/(%s)', ugks_CL.py:298 (C: 1/0 = inf)
-                    wT.append(float(np.float64(1.0) / np.float64(bake_str(bcx.T))))
-                wP.append(bake_str(bcx.P) if bcx.type_of_BC == bcd.OUTFLOW else -1.0)
-            else:
-                wD.append(-1.0); wU.append(-1.0); wV.append(-1.0); wT.append(-1.0); wP.append(-1.0)
-            if bcx.type_of_BC == bcd.ADSORBING:
-                bn.append(bake_e15(bcx.beta_n)); bt.append(bake_e15(bcx.beta_t))
-                gf.append(bake_e15(bcx.gamma_f)); ap.append(bake_e15(bcx.alpha_p))
-                an.append(bake_e15(bcx.alpha_n)); at.append(bake_e15(bcx.alpha_t))
-                dyn.append(int(bcx.dynamic))
-                if np.ndim(bcx.adsorb) != 0:
-                    k = len(cad)
-                    iso[k], iso_tri[k], iso_nbr[k] = isotherm_tables(bcx.adsorb)
-                    cad.append(0); ads.append(0.0)
-                else:
-                    a = float(bcx.adsorb)
-                    if (a > 1.0) | (a <= 0.0):
-                        raise RuntimeError(f"adsorbing wall: equilibrium coverage {a!r} (adsorb / S_T) must lie in (0, 1]")
-                    cad.append(1); ads.append(bake_g(a))
-            else:
-                bn.append(-1.0); bt.append(-1.0); gf.append(-1.0); ap.append(-1.0)
-                an.append(-1.0); at.append(-1.0)
-                dyn.append(-1); cad.append(0); ads.append(0.0)
-        c.update(wall_D=wD, wall_U=wU, wall_V=wV, wall_T=wT, wall_P=wP, beta_n=bn, beta_t=bt,
-                 gamma_f=gf, alpha_p=ap, dynamic_vartheta=dyn, constant_adsorb=cad, adsorb=ads,
-                 alpha_n=an, alpha_t=at, iso=iso, iso_tri=iso_tri, iso_nbr=iso_nbr)
-        return c
+        return block_constants(self, b)
 
 
 # -------------------------------------------------------------- geometry ---
@@ -668,3 +617,83 @@ def initial_macro(b: BlockDef):
         m[:, :, 2] = im[:, :, 2]
         m[:, :, 3] = 1.0 / im[:, :, 3]
     return m, q
+
+
+def ghost_of(gdata) -> int:
+    """ugks_block.py:88-91"""
+    return 3 if gdata.flux_method == "WENO5" else 2
+
+
+def flux_method_code(gdata) -> int:
+    """ugks_CL.py:33-44"""
+    return {"vanLeer": 0, "WENO5": 1, "minmod": 3}[gdata.flux_method]
+
+
+def velocity_tables(gdata):
+    """QUAD/WEIGHT as the kernels see them ('%0.15e', ugks_CL.py:318-346) and the
+    mirror tables.  ``gdata`` is this package's UGKSData or the reference's
+    (the same attribute names: quad, weight, Nv, mirror_*)."""
+    q = np.array([[bake_e15(a), bake_e15(b)] for a, b in gdata.quad], dtype=np.float64)
+    w = np.array([bake_e15(a) for a in gdata.weight], dtype=np.float64)
+    nv = gdata.Nv
+    zeros = np.zeros(nv, dtype=np.int64)
+    mns = gdata.mirror_NS if gdata.mirror_NS is not None else zeros
+    mew = gdata.mirror_EW if gdata.mirror_EW is not None else zeros
+    md = gdata.mirror_D if gdata.mirror_D is not None else zeros
+    return (np.ascontiguousarray(q), np.ascontiguousarray(w), np.ascontiguousarray(mns, dtype=np.int64),
+            np.ascontiguousarray(mew, dtype=np.int64), np.ascontiguousarray(md, dtype=np.int64))
+
+def block_constants(gdata, b) -> dict:
+    """The #define set genHeader emits for one block (ugks_CL.py:58-310).
+    ``gdata`` and ``b`` are this package's UGKSData and BlockDef or the
+    reference's gdata and UGKSBlock: only the attribute names they share are
+    read (flux_method, Nv, relax_type, Pr, Kn_eff, gamma, b, K, omega, CFL,
+    umax, vmax; ni, nj, bc_list and the boundary conditions' fields)."""
+    G = ghost_of(gdata)
+    c = dict(ni=b.ni, nj=b.nj, NI=b.ni + 2 * G, NJ=b.nj + 2 * G, NV=gdata.Nv, ghost=G,
+             flux_method=flux_method_code(gdata), relax_type=int(gdata.relax_type),
+             Pr=bake_str(gdata.Pr), Kn_eff=bake_e15(gdata.Kn_eff), gam=bake_str(gdata.gamma),
+             B=bake_str(gdata.b), K=bake_str(gdata.K), omega=bake_str(gdata.omega),
+             CFL=bake_e15(gdata.CFL), umax=bake_str(gdata.umax), vmax=bake_str(gdata.vmax))
+    wD, wU, wV, wT, wP = [], [], [], [], []
+    bn, bt, gf, ap, dyn, cad, ads = [], [], [], [], [], [], []
+    an, at = [], []
+    iso, iso_tri, iso_nbr = [None] * 4, [None] * 4, [None] * 4
+    for bcx in b.bc_list:
+        if bcx.type_of_BC in (bcd.ADSORBING, bcd.DIFFUSE, bcd.CONSTANT, bcd.INFLOW, bcd.OUTFLOW):
+            wD.append(bake_str(bcx.D))
+            wU.append(bake_str(bcx.U))
+            wV.append(bake_str(bcx.V))
+            with np.errstate(divide="ignore"):       # '1.0/(%s)', ugks_CL.py:298 (C: 1/0 = inf)
+                wT.append(float(np.float64(1.0) / np.float64(bake_str(bcx.T))))
+            wP.append(bake_str(bcx.P) if bcx.type_of_BC == bcd.OUTFLOW else -1.0)
+        else:
+            wD.append(-1.0); wU.append(-1.0); wV.append(-1.0); wT.append(-1.0); wP.append(-1.0)
+        if bcx.type_of_BC == bcd.ADSORBING:
+            bn.append(bake_e15(bcx.beta_n)); bt.append(bake_e15(bcx.beta_t))
+            gf.append(bake_e15(bcx.gamma_f)); ap.append(bake_e15(bcx.alpha_p))
+            an.append(bake_e15(bcx.alpha_n)); at.append(bake_e15(bcx.alpha_t))
+            dyn.append(int(bcx.dynamic))
+            if np.ndim(bcx.adsorb) != 0:
+                k = len(cad)
+                iso[k], iso_tri[k], iso_nbr[k] = isotherm_tables(bcx.adsorb)
+                cad.append(0); ads.append(0.0)
+            else:
+                a = float(bcx.adsorb)
+                if (a > 1.0) | (a <= 0.0):
+                    raise RuntimeError(f"adsorbing wall: equilibrium coverage {a!r} (adsorb / S_T) must lie in (0, 1]")
+                cad.append(1); ads.append(bake_g(a))
+        else:
+            bn.append(-1.0); bt.append(-1.0); gf.append(-1.0); ap.append(-1.0)
+            an.append(-1.0); at.append(-1.0)
+            dyn.append(-1); cad.append(0); ads.append(0.0)
+    c.update(wall_D=wD, wall_U=wU, wall_V=wV, wall_T=wT, wall_P=wP, beta_n=bn, beta_t=bt,
+             gamma_f=gf, alpha_p=ap, dynamic_vartheta=dyn, constant_adsorb=cad, adsorb=ads,
+             alpha_n=an, alpha_t=at, iso=iso, iso_tri=iso_tri, iso_nbr=iso_nbr)
+    # wall values given as expressions, on the walls that carry wall values (ugks_CL.py:278-298)
+    from .udf import block_programs, is_time_dependent
+    carry = [bcx if bcx.type_of_BC in (bcd.ADSORBING, bcd.DIFFUSE, bcd.CONSTANT, bcd.INFLOW, bcd.OUTFLOW)
+             else None for bcx in b.bc_list]
+    c["udf"] = block_programs(carry) if any(x is not None and x.has_udf() for x in carry) else None
+    c["time_dependent_BC"] = any(is_time_dependent(bcx) for bcx in b.bc_list)
+    return c

@@ -20,6 +20,61 @@ import numpy as np
 from . import lib as L
 
 
+def plan_inputs(sims, precision, stream=None):
+    """The C-ABI inputs of a step over the blocks of ``sims`` (cases): block
+    handles, buffer records (with the FP32 copies of f and of the face fluxes
+    allocated and filled for the mixed step), boundary records, the case of
+    each block and one time-step record per case."""
+    lib = L.load()
+    mixed = precision == "mixed"
+    blocks = [(ci, b) for ci, sim in enumerate(sims) for b in sim.blocks]
+    first = np.cumsum([0] + [len(sim.blocks) for sim in sims])
+    nb = len(blocks)
+    bufs = (L.BlockBuffers * nb)()
+    bcs = (L.BCDesc * (4 * nb))()
+    handles = (C.c_void_p * nb)()
+    case_of = (C.c_int32 * nb)(*[ci for ci, _ in blocks])
+    rt = {"S": 0, "D": 1, "CL": 2}
+    for bi, (ci, b) in enumerate(blocks):
+        b.macro_new_D = L.DeviceArray((b.ni, b.nj, 4), fill=0.0)
+        if mixed:
+            dist = (b.Ni, b.Nj, b.Nv, 2)
+            b.f32_D = L.DeviceArray(dist, np.float32, fill=0.0)
+            b.fluxS32_D = L.DeviceArray(dist, np.float32, fill=0.0)
+            b.fluxW32_D = L.DeviceArray(dist, np.float32, fill=0.0)
+            n = int(np.prod(dist))
+            L.check(lib.ugks_f64_to_f32(stream, b.f_D.ptr, b.f32_D.ptr, n), "f64_to_f32")
+            f, fs, fw = b.f32_D, b.fluxS32_D, b.fluxW32_D
+        else:
+            f, fs, fw = b.f_D, b.flux_f_S_D, b.flux_f_W_D
+        u = bufs[bi]
+        u.f, u.flux_f_S, u.flux_f_W = f.ptr, fs.ptr, fw.ptr
+        u.flux_macro_S, u.flux_macro_W = b.flux_macro_S_D.ptr, b.flux_macro_W_D.ptr
+        u.macro, u.macro_new, u.residual, u.Q = b.macro_D.ptr, b.macro_new_D.ptr, b.residual_D.ptr, b.Q_D.ptr
+        u.xy, u.centre, u.mid_side, u.normal = b.xy_D.ptr, b.centre_D.ptr, b.side_D.ptr, b.normal_D.ptr
+        u.area, u.side_length = b.area_D.ptr, b.length_D.ptr
+        u.wall_prop, u.wall_cover, u.wall_fluxes = b.wall_prop_D.ptr, b.wall_cover_D.ptr, b.wall_fluxes_D.ptr
+        u.wall_dist, u.time_step, u.flag = b.wall_dist_D.ptr, b.time_step_D.ptr, b.flag_D.ptr
+        u.para = b.para_D.ptr
+        handles[bi] = b.prg.handle
+        for k, x in enumerate(b.bc_list):
+            other = int(x.other_block)
+            if other >= 0:
+                other += int(first[ci])
+            bcs[4 * bi + k] = L.BCDesc(int(x.type_of_BC), other, int(x.other_face),
+                                       int(x.orientation), int(x.flip_distribution), rt.get(x.reflect_type, 0))
+    dts = (L.DtDesc * len(sims))()
+    for ci, sim in enumerate(sims):
+        g = sim.gdata
+        # the blocks' maximum time steps of the last calcTimeStep: the first
+        # step re-evaluates dt against them, as UGKSim.one_step does
+        md = [float(getattr(b, "max_dt", 0.0) or 0.0) for b in sim.blocks]
+        dts[ci] = L.DtDesc(g.dt, g.time, g.CFL, g.delta_dt, g.suggest_dt if g.suggest_dt else 0.0, g.Kn_eff,
+                           int(sim.step), 0, int(bool(getattr(sim, "force_time", False))), 0,
+                           min(md) if all(m > 0 for m in md) else 0.0)
+    return blocks, bufs, bcs, handles, case_of, dts
+
+
 class StepPlan:
     KERNELS = ("k_faceA", "k_faceS", "k_faceC", "k_wall", "k_update", "k_dt", "k_dtfin", "k_ghost")
 
@@ -34,46 +89,8 @@ class StepPlan:
         if precision not in ("fp64", "mixed"):
             raise ValueError("precision must be 'fp64' or 'mixed'")
         lib = L.load()
-        blocks = [(ci, b) for ci, sim in enumerate(self.sims) for b in sim.blocks]
-        first = np.cumsum([0] + [len(sim.blocks) for sim in self.sims])
+        blocks, bufs, bcs, handles, case_of, dts = plan_inputs(self.sims, precision, stream)
         nb = len(blocks)
-        bufs = (L.BlockBuffers * nb)()
-        bcs = (L.BCDesc * (4 * nb))()
-        handles = (C.c_void_p * nb)()
-        case_of = (C.c_int32 * nb)(*[ci for ci, _ in blocks])
-        rt = {"S": 0, "D": 1, "CL": 2}
-        for bi, (ci, b) in enumerate(blocks):
-            b.macro_new_D = L.DeviceArray((b.ni, b.nj, 4), fill=0.0)
-            if mixed:
-                dist = (b.Ni, b.Nj, b.Nv, 2)
-                b.f32_D = L.DeviceArray(dist, np.float32, fill=0.0)
-                b.fluxS32_D = L.DeviceArray(dist, np.float32, fill=0.0)
-                b.fluxW32_D = L.DeviceArray(dist, np.float32, fill=0.0)
-                n = int(np.prod(dist))
-                L.check(lib.ugks_f64_to_f32(stream, b.f_D.ptr, b.f32_D.ptr, n), "f64_to_f32")
-                f, fs, fw = b.f32_D, b.fluxS32_D, b.fluxW32_D
-            else:
-                f, fs, fw = b.f_D, b.flux_f_S_D, b.flux_f_W_D
-            u = bufs[bi]
-            u.f, u.flux_f_S, u.flux_f_W = f.ptr, fs.ptr, fw.ptr
-            u.flux_macro_S, u.flux_macro_W = b.flux_macro_S_D.ptr, b.flux_macro_W_D.ptr
-            u.macro, u.macro_new, u.residual, u.Q = b.macro_D.ptr, b.macro_new_D.ptr, b.residual_D.ptr, b.Q_D.ptr
-            u.xy, u.centre, u.mid_side, u.normal = b.xy_D.ptr, b.centre_D.ptr, b.side_D.ptr, b.normal_D.ptr
-            u.area, u.side_length = b.area_D.ptr, b.length_D.ptr
-            u.wall_prop, u.wall_cover, u.wall_fluxes = b.wall_prop_D.ptr, b.wall_cover_D.ptr, b.wall_fluxes_D.ptr
-            u.wall_dist, u.time_step, u.flag = b.wall_dist_D.ptr, b.time_step_D.ptr, b.flag_D.ptr
-            handles[bi] = b.prg.handle
-            for k, x in enumerate(b.bc_list):
-                other = int(x.other_block)
-                if other >= 0:
-                    other += int(first[ci])
-                bcs[4 * bi + k] = L.BCDesc(int(x.type_of_BC), other, int(x.other_face),
-                                           int(x.orientation), int(x.flip_distribution), rt.get(x.reflect_type, 0))
-        dts = (L.DtDesc * len(self.sims))()
-        for ci, sim in enumerate(self.sims):
-            g = sim.gdata
-            dts[ci] = L.DtDesc(g.dt, g.time, g.CFL, g.delta_dt, g.suggest_dt if g.suggest_dt else 0.0, g.Kn_eff,
-                               int(sim.step), 0, int(bool(getattr(sim, "force_time", False))), 0)
         h = C.c_void_p()
         L.check(lib.ugks_step_plan_create_batch(handles, bufs, bcs, nb, case_of, len(self.sims),
                                                 L.UGKS_MIXED if mixed else L.UGKS_FP64, dts, C.byref(h)),
@@ -148,3 +165,4 @@ class StepPlan:
             self.close()
         except Exception:
             pass
+

@@ -88,7 +88,7 @@ class BlockBuffers(C.Structure):
     _fields_ = [(n, P) for n in (
         "f", "flux_f_S", "flux_f_W", "flux_macro_S", "flux_macro_W", "macro", "macro_new", "residual", "Q",
         "xy", "centre", "mid_side", "normal", "area", "side_length", "wall_prop", "wall_cover", "wall_fluxes",
-        "wall_dist", "time_step", "flag")]
+        "wall_dist", "time_step", "flag", "para")]
 
 
 class BCDesc(C.Structure):
@@ -98,7 +98,7 @@ class BCDesc(C.Structure):
 
 class DtDesc(C.Structure):
     _fields_ = [("dt", D), ("time", D), ("CFL", D), ("delta_dt", D), ("suggest_dt", D), ("Kn_eff", D),
-                ("step", I), ("overshoot", I), ("force_time", I), ("pad", I)]
+                ("step", I), ("overshoot", I), ("force_time", I), ("pad", I), ("max_dt", D)]
 
 RUNTIME = {
     "ugks_block_create": [P, C.POINTER(P)],
@@ -138,6 +138,8 @@ class BlockDesc(C.Structure):
         ("alpha_n", D * 4), ("alpha_t", D * 4),
         ("iso", C.c_void_p * 4), ("iso_tri", C.c_void_p * 4), ("iso_nbr", C.c_void_p * 4),
         ("n_iso", I * 4), ("n_tri", I * 4),
+        ("udf_code", C.c_void_p), ("udf_const", C.c_void_p), ("udf_ncode", I), ("udf_nconst", I),
+        ("udf_start", I * 16), ("udf_len", I * 16),
     ]
 
 
@@ -330,6 +332,14 @@ def make_desc(consts: dict, tables) -> tuple:
             d.iso[k], d.iso_tri[k], d.iso_nbr[k] = pts.ctypes.data, tri.ctypes.data, nbr.ctypes.data
             d.n_iso[k], d.n_tri[k] = len(pts), len(tri)
             keep += [pts, tri, nbr]
+    udf = consts.get("udf")
+    if udf is not None:
+        code, cst, start, length = udf
+        d.udf_code, d.udf_const = code.ctypes.data, cst.ctypes.data
+        d.udf_ncode, d.udf_nconst = len(code), len(cst)
+        d.udf_start = (I * 16)(*start)
+        d.udf_len = (I * 16)(*length)
+        keep += [code, cst]
     return d, tuple(keep)
 
 

@@ -17,7 +17,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def header_functions():
     txt = open(os.path.join(ROOT, "include", "ugks_hip.h")).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-    return sorted(set(re.findall(r"\b(?:int|const char \*)\s*(ugks_\w+)\s*\(", txt)))
+    return sorted(set(re.findall(r"\b(?:int|size_t|const char \*)\s*(ugks_\w+)\s*\(", txt)))
 
 
 def test_library_exports_every_declared_symbol():
@@ -32,10 +32,11 @@ def test_library_exports_every_declared_symbol():
 
 def test_descriptor_layouts():
     # ugks_block_desc: 8 ints, 9 doubles, 5x4 doubles, 4x4 doubles, 2x4 ints, 4 doubles, 5 pointers,
-    # 2x4 doubles, 3x4 pointers, 2x4 ints
+    # 2x4 doubles, 3x4 pointers, 2x4 ints; wall-value programs: 2 pointers, 2 ints, 2x16 ints
     expect = 8 * 4 + 9 * 8 + 20 * 8 + 16 * 8 + 8 * 4 + 4 * 8 + 5 * 8 + 8 * 8 + 12 * 8 + 8 * 4
+    expect += 2 * 8 + 2 * 4 + 32 * 4
     assert C.sizeof(lib.BlockDesc) == expect
-    assert C.sizeof(lib.BlockBuffers) == 21 * 8
+    assert C.sizeof(lib.BlockBuffers) == 22 * 8
     assert C.sizeof(lib.BCDesc) == 6 * 4
     assert C.sizeof(lib.DtDesc) == 6 * 8 + 4 * 4 + 8
 
@@ -170,3 +171,28 @@ def test_over_wide_block_is_refused_before_any_launch():
     assert L.ugks_step_block_fits(nj_max + 1, 2401, WENO5, lib.UGKS_MIXED) == 0
     assert L.ugks_step_block_fits(5 * nj_max // 3, 2401, VANLEER, lib.UGKS_FP64) == 0
     assert L.ugks_step_block_fits(0, 2401, WENO5, lib.UGKS_FP64) == -1
+
+
+def test_block_constants_from_objects_with_the_reference_attribute_names():
+    """The drop-in binding builds a block's descriptor from the reference's own
+    gdata and UGKSBlock: only attribute names those objects have are read."""
+    import types
+
+    from pyugks_amd import cases
+    from pyugks_amd.data import block_constants, velocity_tables
+    g = cases.cavity(6, ("NC", 8), "WENO5", 0.05, n_cells_j=4).prepare()
+    ref_g = types.SimpleNamespace(**{k: getattr(g, k) for k in (
+        "flux_method", "Nv", "relax_type", "Pr", "Kn_eff", "gamma", "b", "K", "omega", "CFL", "umax", "vmax",
+        "quad", "weight", "mirror_NS", "mirror_EW", "mirror_D")})
+    blk = g.blocks[0]
+    ref_b = types.SimpleNamespace(ni=blk.ni, nj=blk.nj, bc_list=blk.bc_list)
+    a, b = block_constants(ref_g, ref_b), g.block_constants(blk)
+    assert set(a) == set(b)
+    for k in a:
+        if isinstance(a[k], list) and a[k] and isinstance(a[k][0], np.ndarray):
+            continue
+        assert a[k] == b[k] or (a[k] is None and b[k] is None), k
+    for x, y in zip(velocity_tables(ref_g), g.velocity_tables()):
+        assert np.array_equal(x, y)
+    desc, keep = lib.make_desc(a, velocity_tables(ref_g))
+    assert desc.NV == g.Nv and desc.ghost == 3

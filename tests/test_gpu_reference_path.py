@@ -112,6 +112,20 @@ def small_cases():
                 if bcx.type_of_BC == bcd.ADJACENT:
                     bcx.flip_distribution = flip
         out[name] = g
+    # wall values given as expressions (UDF strings): a lid whose speed varies in time and along
+    # the wall, walls whose temperature varies along them, and a constant-state inlet in time
+    g = cases.cavity(8, ("NC", 8), "WENO5", 0.05, lid=0.15, n_cells_j=6)
+    g.blocks[0].bc_list[bcd.NORTH] = bcd.DiffuseBC(D=1.0, T=1.0, UDF_U="0.2*sin(2*PI*t/0.05) + 0.05*s",
+                                                   UDF_T="1.0 + 0.1*x")
+    g.blocks[0].bc_list[bcd.SOUTH] = bcd.DiffuseBC(D=1.0, T=1.0, UDF_T="0.9 + 0.2*s*s")
+    out["cavity_udf_walls"] = g
+    g = cases._base("vanLeer", 0.05)
+    cases.newton_cotes(g, 8)
+    bcs = [bcd.DiffuseBC(D=1.0, T=1.0), bcd.ExtrapolateOutBC(), bcd.DiffuseBC(D=1.0, T=1.0),
+           bcd.ConstantBC(D=1.0, T=1.0, UDF_U="0.3*(1.0 + 4*t)", UDF_D="1.0 + 0.05*sin(PI*y)")]
+    g.blocks = cartesian_superblock(0.0, 2.0, 0.0, 1.0, 12, 6, bcs, FlowCondition(D=1.0, U=0.3, V=0.0, T=1.0), nbj=1,
+                                    i_splits=[5, 7])
+    out["channel_udf_inlet"] = g
     return out
 
 
