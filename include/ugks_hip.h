@@ -108,6 +108,10 @@ int ugks_initFunctions(ugks_block *blk, void *stream, void *Fin, const void *mac
 int ugks_calcMacro(ugks_block *blk, void *stream, const void *Fin, void *macro);
 /* clKernel.c:480-483, ugks_block.py:1271-1272 */
 int ugks_calcQ(ugks_block *blk, void *stream, const void *Fin, const void *macro, void *gQ);
+/* getInternal (clKernel.c:315-372): Txyz [ni][nj] double4 (Tx, Ty, Tz, T) and
+ * Stress [ni][nj] double4 (tau_xx, tau_xy, tau_yy, 0) from f, the diagnostics
+ * of save_options.internal_data (ugks_block.py:1335-1348) */
+int ugks_getInternal(ugks_block *blk, void *stream, const void *Fin, void *Txyz, void *Stress);
 /* clUGKS.c:110-117, ugks_block.py:1066-1068 */
 int ugks_iFace(ugks_block *blk, void *stream, const void *Fin, void *iface_f, void *fsigma,
                const void *centre, const void *mid_side, const void *normal, int32_t face);
@@ -302,7 +306,6 @@ int ugks_copy_kernel(void *stream, const void *src, void *dst, int64_t bytes);
 /* element-wise precision conversion of n values (f of the mixed-precision step) */
 int ugks_f64_to_f32(void *stream, const void *src, void *dst, int64_t n);
 int ugks_f32_to_f64(void *stream, const void *src, void *dst, int64_t n);
-
 
 #ifdef __cplusplus
 }

@@ -90,6 +90,7 @@ def lib():
         _lib.or_sim_one_step.argtypes = [C.POINTER(Sim), C.c_int]
         _lib.or_sim_run.argtypes = [C.POINTER(Sim), C.c_int]
         _lib.or_set_threads.argtypes = [C.c_int]
+        _lib.or_getInternal.argtypes = [C.c_void_p] * 4
         for n in ("or_sim_init", "or_sim_one_step", "or_sim_run"):
             getattr(_lib, n).restype = C.c_int
     return _lib
@@ -196,6 +197,15 @@ class OracleBlock:
     def __getitem__(self, name):
         return self.arrays[name]
 
+    def internal(self):
+        """getInternal (clKernel.c:315-372): (Tx, Ty, Tz, T) and (tau_xx,
+        tau_xy, tau_yy, 0) per interior cell of the block's current f."""
+        c = self.consts
+        txyz, stress = np.zeros((c["ni"], c["nj"], 4)), np.zeros((c["ni"], c["nj"], 4))
+        lib().or_getInternal(C.byref(self.block.p), self.arrays["f"].ctypes.data, txyz.ctypes.data,
+                             stress.ctypes.data)
+        return txyz, stress
+
 
 # ---- wall values given as expressions (UDF_D/U/V/T), restated ------------
 # The reference pastes each UDF string into the OpenCL source as a #define
@@ -241,7 +251,8 @@ def udf_values(text: str, s, x, y, t):
                 return _UDF_FUNCS[n.func.id](*args), False
         raise ValueError(f"unsupported UDF syntax: {ast.dump(n)[:60]}")
 
-    return np.broadcast_to(np.float64(ev(ast.parse(text.strip(), mode="eval"))[0]), np.shape(s)).astype(np.float64)
+    v = np.asarray(ev(ast.parse(text.strip(), mode="eval"))[0], dtype=np.float64)
+    return np.broadcast_to(v, np.shape(s)).astype(np.float64)
 
 
 def wall_para(ni, nj, G, side_length, x, y):

@@ -398,6 +398,30 @@ void or_calcQ(const or_params *P, const od2 *Fin, const od4 *macro, od2 *gQ) {
         }
 }
 
+/* getInternal, clKernel.c:315-372 */
+void or_getInternal(const or_params *P, const od2 *Fin, od4 *Txyz, od4 *Stress) {
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+    for (int mi = 0; mi < P->ni; mi++)
+        for (int mj = 0; mj < P->nj; mj++) {
+            double D = 0.0, Su = 0.0, Sv = 0.0, Suu = 0.0, Svv = 0.0, Suv = 0.0, H = 0.0;
+            for (int gv = 0; gv < NV_; gv++) {
+                od2 uv = quad_at(P, gv), f = F3(Fin, mi + G_, mj + G_, gv);
+                D += f.x;
+                Su += uv.x * f.x;
+                Sv += uv.y * f.x;
+                Suu += uv.x * uv.x * f.x;
+                Svv += uv.y * uv.y * f.x;
+                Suv += uv.x * uv.y * f.x;
+                H += f.y;
+            }
+            double U = Su / D, V = Sv / D;
+            double Tx = 2 * (Suu / D - U * U), Ty = 2 * (Svv / D - V * V), Tz = 2 * (H / D);
+            double T = (Tx + Ty + Tz) / P->B;
+            M2(Txyz, mi, mj) = v4(Tx, Ty, Tz, T);
+            M2(Stress, mi, mj) = v4(Suu - D * (U * U + T / 2.0), Suv - D * (U * V + T / 2.0), Svv - D * (V * V + T / 2.0), 0.0);
+        }
+}
+
 /* iFace, clUGKS.c:110-199 */
 void or_iFace(const or_params *P, const od2 *Fin, od2 *iface_f, od2 *fsigma,
               const od2 *centre, const od2 *mid_side, const od2 *normal, int face) {

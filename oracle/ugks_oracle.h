@@ -111,6 +111,7 @@ void or_clFindDT(const or_params *P, const od2 *xy, const double *area,
 void or_initFunctions(const or_params *P, od2 *Fin, const od4 *macro, const od2 *gQ);
 void or_calcMacro(const or_params *P, const od2 *Fin, od4 *macro);
 void or_calcQ(const or_params *P, const od2 *Fin, const od4 *macro, od2 *gQ);
+void or_getInternal(const or_params *P, const od2 *Fin, od4 *Txyz, od4 *Stress);
 void or_iFace(const or_params *P, const od2 *Fin, od2 *iface_f, od2 *fsigma,
               const od2 *centre, const od2 *mid_side, const od2 *normal, int face);
 void or_getFaceCons(const or_params *P, const od2 *iface_f, const od2 *normal, int face,

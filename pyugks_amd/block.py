@@ -345,6 +345,41 @@ class UGKSBlock:
         self.wall_fluxes_H = self.wall_fluxes_D.download(s)
         self.wall_prop_H = self.wall_prop_D.download(s)
 
+    def getInternal(self):
+        """ugks_block.py:1334-1348: Txyz_H (Tx, Ty, Tz, T) and stress_H
+        (tau_xx, tau_xy, tau_yy, 0) per cell, when save_options.internal_data."""
+        if not self.gdata.save_options.internal_data:
+            return None
+        s = self.stream
+        if getattr(self, "Txyz_D", None) is None:
+            self.Txyz_D = L.DeviceArray((self.ni, self.nj, 4), fill=0.0)
+            self.stress_D = L.DeviceArray((self.ni, self.nj, 4), fill=0.0)
+        self.prg.getInternal(s, self.f_D, self.Txyz_D, self.stress_D)
+        self.Txyz_H = self.Txyz_D.download(s)
+        self.stress_H = self.stress_D.download(s)
+        return self.Txyz_H, self.stress_H
+
+    def get_interface_distribution(self):
+        """ugks_block.py:985-1041: the reconstructed distribution at every wall
+        face (iFace into the face buffers, whose contents are put back after):
+        wall_f_N/E/S/W, [wall length][Nv][2]."""
+        if not self.gdata.save_options.save_final_wall_distribution:
+            return
+        s, G = self.stream, self.ghost
+        keep_S, keep_W = self.flux_f_S_D.download(s), self.flux_f_W_D.download(s)
+        self.prg.iFace(s, self.f_D, self.flux_f_S_D, self.sigma_D, self.centre_D, self.side_D, self.normal_D,
+                       np.int32(SOUTH))
+        fS = self.flux_f_S_D.download(s)
+        self.wall_f_N = np.copy(fS[G:self.Ni - G, self.Nj - G])
+        self.wall_f_S = np.copy(fS[G:self.Ni - G, G])
+        self.prg.iFace(s, self.f_D, self.flux_f_W_D, self.sigma_D, self.centre_D, self.side_D, self.normal_D,
+                       np.int32(WEST))
+        fW = self.flux_f_W_D.download(s)
+        self.wall_f_E = np.copy(fW[self.Ni - G, G:self.Nj - G])
+        self.wall_f_W = np.copy(fW[G, G:self.Nj - G])
+        self.flux_f_S_D.upload(keep_S, s)
+        self.flux_f_W_D.upload(keep_W, s)
+
     def updateHost(self, get_f=False, get_wall_f=False):
         """ugks_block.py:1350-1412: host copies of the macro and wall state
         (macro_H, wall_cover_H, wall_fluxes_H, wall_prop_H) and of the heat

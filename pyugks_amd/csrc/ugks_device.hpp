@@ -38,6 +38,7 @@ __host__ __device__ inline double length(d2 a) { return sqrt(a.x * a.x + a.y * a
 struct Consts {
     int ni, nj, NI, NJ, NV, G, method, relax;
     double Pr, Kn, gam, K, omega, umax, vmax;
+    double B;                        // b = 2 / (gamma - 1), baked (ugks_CL.py:86-93)
     double wall_D[4], wall_U[4], wall_V[4], wall_T[4], wall_P[4];
     double beta_n[4], beta_t[4], gamma_f[4], alpha_p[4], adsorb[4];
     double alpha_n[4], alpha_t[4];   // Cercignani-Lampis accommodation (ALPHA_N/ALPHA_T, ugks_CL.py:134-148)

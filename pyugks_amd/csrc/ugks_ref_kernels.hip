@@ -172,6 +172,35 @@ __global__ void k_calcQ(Consts C, const d2 *F, const d4 *macro, d2 *gQ) {
     if (lane == 0) gQ[midx(C, mi, mj)] = mk2(qx, qy);
 }
 
+// getInternal (clKernel.c:315-372): the translational temperatures Tx, Ty, Tz
+// and T = (Tx + Ty + Tz) / B, and the stress (tau_xx, tau_xy, tau_yy, 0), from
+// the moments of f; a diagnostic the HDF5 output writes (ugks_block.py:1335-1348)
+__global__ void k_getInternal(Consts C, const d2 *F, d4 *Txyz, d4 *Stress) {
+    long long w = wave_id();
+    if (w >= (long long)C.ni * C.nj) return;
+    int mi = (int)(w / C.nj), mj = (int)(w % C.nj), lane = lane_id();
+    double m[8] = {0, 0, 0, 0, 0, 0, 0, 0};      // D, M1, M2, M4, M5, M6, H
+    for (int v = lane; v < C.NV; v += 64) {
+        const d2 f = F[fidx(C, mi + C.G, mj + C.G, v)], uv = C.quad[v];
+        m[0] += f.x;
+        m[1] += uv.x * f.x;
+        m[2] += uv.y * f.x;
+        m[3] += uv.x * uv.x * f.x;
+        m[4] += uv.y * uv.y * f.x;
+        m[5] += uv.x * uv.y * f.x;
+        m[6] += f.y;
+    }
+#pragma unroll
+    for (int q = 0; q < 7; q++) m[q] = wave_sum(m[q]);
+    if (lane != 0) return;
+    const double D = m[0], U = m[1] / D, V = m[2] / D;
+    const double Tx = 2 * (m[3] / D - U * U), Ty = 2 * (m[4] / D - V * V), Tz = 2 * (m[6] / D);
+    const double T = (Tx + Ty + Tz) / C.B;
+    Txyz[midx(C, mi, mj)] = mk4(Tx, Ty, Tz, T);
+    Stress[midx(C, mi, mj)] = mk4(m[3] - D * (U * U + T / 2.0), m[5] - D * (U * V + T / 2.0),
+                                  m[4] - D * (V * V + T / 2.0), 0.0);
+}
+
 // ---------------------------------------------------------------- faces
 // iFace (clUGKS.c:110-199): upwind reconstruction of f and its slope at a face
 __global__ void k_iFace(Consts C, const d2 *F, d2 *iface_f, d2 *fsigma, const d2 *centre, const d2 *mid_side,
@@ -871,6 +900,14 @@ int ugks_calcQ(ugks_block *blk, void *stream, const void *Fin, const void *macro
     hipLaunchKernelGGL(k_calcQ, dim3(blocks_for_waves((long long)C.ni * C.nj)), dim3(kBlock), 0, S(stream), C,
                        (const d2 *)Fin, (const d4 *)macro, (d2 *)gQ);
     return ugks_rt::check_launch("calcQ");
+}
+
+int ugks_getInternal(ugks_block *blk, void *stream, const void *Fin, void *Txyz, void *Stress) {
+    UGKS_BLK_CHECK(blk);
+    const Consts &C = blk->c;
+    hipLaunchKernelGGL(k_getInternal, dim3(blocks_for_waves((long long)C.ni * C.nj)), dim3(kBlock), 0, S(stream), C,
+                       (const d2 *)Fin, (d4 *)Txyz, (d4 *)Stress);
+    return ugks_rt::check_launch("getInternal");
 }
 
 int ugks_iFace(ugks_block *blk, void *stream, const void *Fin, void *iface_f, void *fsigma, const void *centre,

@@ -170,6 +170,7 @@ int ugks_block_create(const ugks_block_desc *d, ugks_block **out) {
     c.method = d->flux_method; c.relax = d->relax_type;
     c.Pr = d->Pr; c.Kn = d->Kn_eff; c.gam = d->gam; c.K = d->K; c.omega = d->omega;
     c.umax = d->umax; c.vmax = d->vmax;
+    c.B = d->B;
     for (int k = 0; k < 4; k++) {
         c.wall_D[k] = d->wall_D[k]; c.wall_U[k] = d->wall_U[k]; c.wall_V[k] = d->wall_V[k];
         c.wall_T[k] = d->wall_T[k]; c.wall_P[k] = d->wall_P[k];

@@ -143,6 +143,7 @@ class SaveOptions:
     save_f_always: bool = False
     save_final_flux: bool = False
     save_final_wall_distribution: bool = False
+    internal_data: bool = False      # translational temperatures and stress (getInternal) in each save
     compression: object = "gzip"
     save_count: int = 1
     initial_save_count: object = None

@@ -31,6 +31,7 @@ KERNELS = {
     "initFunctions": [P, P, P],
     "calcMacro": [P, P],
     "calcQ": [P, P, P],
+    "getInternal": [P, P, P],
     "iFace": [P, P, P, P, P, P, I],
     "getFaceCons": [P, P, I, P, I, I],
     "getAL": [P, P, I, P, I, I],

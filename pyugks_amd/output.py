@@ -11,9 +11,7 @@ reads such a run back the way UGKSim.restart and UGKSBlock.initFunctions do
 
 Differences from the reference, all in what is stored rather than where:
 the block's "CL_source" holds a description of the HIP library (there is no
-OpenCL source), the internal-data fields (Txyz, stress) and the wall
-distributions of the reference's diagnostic kernels are not written, and the
-global "device" / "platform" name the HIP device.  Needs the HDF5 C library
+OpenCL source), and the global "device" / "platform" name the HIP device.  Needs the HDF5 C library
 (pyugks_amd.h5)."""
 from __future__ import annotations
 
@@ -142,7 +140,7 @@ class HDFOutput:
         bcg.close()
         blk.close()
 
-    def save(self, save_f: bool = False, save_flux: bool = False) -> str:
+    def save(self, save_f: bool = False, save_flux: bool = False, save_wall_f: bool = False) -> str:
         """One step file (saveHDF5, ugks_sim.py:1021-1064); returns its path."""
         sim, g, c = self.sim, self.sim.gdata, self.compression
         width = int(max(math.log10(max(g.max_step, 1)), 3))
@@ -154,14 +152,14 @@ class HDFOutput:
         with h5.File(path, "w") as f:
             f.create_dataset("step", int(sim.step))
             for b in sim.blocks:
-                xd = self._save_block(f, b, short, save_f, save_flux)
+                xd = self._save_block(f, b, short, save_f, save_flux, save_wall_f)
                 self.xdmf.write(self.xdmf_blocks[b.id])
                 self.xdmf.write(xd)
                 self.xdmf.write("</Grid>\n")
         self.xdmf.write("</Grid>\n")
         return path
 
-    def _save_block(self, f, b, short, save_f, save_flux):
+    def _save_block(self, f, b, short, save_f, save_flux, save_wall_f=False):
         g, c, s = self.sim.gdata, self.compression, b.stream
         ni, nj, G = b.ni, b.nj, b.ghost
         grp = f.create_group("block_%d" % b.id)
@@ -182,6 +180,14 @@ class HDFOutput:
               '<DataItem Dimensions="%d %d" Function="$0*$1/2.0" ItemType="Function">\n' % (ni, nj) +
               _cell_item(short, b.id, "rho", ni, nj) + _cell_item(short, b.id, "T", ni, nj) +
               '</DataItem>\n</Attribute>\n']
+        if g.save_options.internal_data:
+            # getInternal (clKernel.c:315-372), written as in ugks_block.py:1493-1507
+            Txyz, stress = b.getInternal()
+            grp.create_dataset("Txyz", np.ascontiguousarray(Txyz[:, :, 0:-1]), compression=c)
+            grp.create_dataset("stress", np.ascontiguousarray(stress[:, :, 0:3]), compression=c)
+            for name in ("Txyz", "stress"):
+                xd.append('<Attribute Name="%s" AttributeType="Vector" Center="Cell">\n' % name +
+                          _cell_item(short, b.id, name, ni, nj, " 3") + '</Attribute>\n')
         prop = b.wall_prop_D.download(s)
         cover = b.wall_cover_D.download(s)
         wflux = b.wall_fluxes_D.download(s)
@@ -205,6 +211,10 @@ class HDFOutput:
                 data = {0: fS[G:b.Ni - G, b.Nj - G], 1: fW[b.Ni - G, G:b.Nj - G], 2: fS[G:b.Ni - G, G],
                         3: fW[G, G:b.Nj - G]}[fid]
                 w.create_dataset("flux", np.ascontiguousarray(data), compression=c)
+            if save_wall_f:
+                # the reconstructed wall distribution (get_interface_distribution, ugks_block.py:1575-1588)
+                w.create_dataset("wall_f", np.ascontiguousarray((b.wall_f_N, b.wall_f_E, b.wall_f_S, b.wall_f_W)[fid]),
+                                 compression=c)
             w.close()
         wall.close()
         if save_f:

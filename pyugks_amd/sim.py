@@ -142,17 +142,17 @@ class UGKSim:
         return self.output
 
     def saveHDF5(self, save_f=False, save_flux=False, save_wall_f=False):
-        """ugks_sim.py:1022-1063; save_wall_f (the reference's diagnostic
-        wall distributions) is accepted and not written.  While a mixed
-        precision plan steps the run, f lives in FP32: it is widened into the
-        blocks' FP64 arrays first (the saved heat flux is computed from it)."""
+        """ugks_sim.py:1022-1063 (save_wall_f writes the wall distributions
+        get_interface_distribution took).  While a mixed precision plan steps
+        the run, f lives in FP32: it is widened into the blocks' FP64 arrays
+        first (the saved heat flux and internal data are computed from it)."""
         if self.output is None:
             self.initHDF()
         plan = getattr(self, "_plan", None)
         if plan is not None and plan.precision == "mixed":
             for b in self.blocks:
                 b.f_D.upload(plan.f64(b), self.stream)
-        return self.output.save(save_f=save_f, save_flux=save_flux)
+        return self.output.save(save_f=save_f, save_flux=save_flux, save_wall_f=save_wall_f)
 
     def close_HDF5(self):
         """ugks_sim.py:1065-1096"""
@@ -413,7 +413,14 @@ class UGKSim:
         if self.exit_reason is None:
             self.exit_reason = "step count reached"
         self.wall_time = time.time() - t0
-        # the final save closes the run's files (ugks_sim.py:693-694)
+        # the reconstructed wall distributions, then the final save that closes the run's files
+        # (ugks_sim.py:692-698); a mixed-precision plan's f is widened into the blocks' FP64 f first
+        if save.save_final_wall_distribution:
+            plan = getattr(self, "_plan", None)
+            for b in self.blocks:
+                if plan is not None and plan.precision == "mixed":
+                    b.f_D.upload(plan.f64(b), self.stream)
+                b.get_interface_distribution()
         self.saveToFile(close_file=True, save_f=save.save_final_f, save_wall_flux=save.save_final_flux,
                         save_wall_f=save.save_final_wall_distribution)
         return self.exit_reason in ("step count reached", "time limit reached", "residual stabilised",

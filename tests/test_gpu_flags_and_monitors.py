@@ -127,7 +127,7 @@ def test_reference_call_form_drives_the_kernels(hip):
     """lib.CLCompatProgram takes the pyopencl form prg.<kernel>(queue,
     global_size, local_size, *args) with the reference's buffers, and the
     kernels so driven reproduce the oracle's calcMacro, calcQ, iFace,
-    getFaceCons and clFindDT on the oracle's initial state."""
+    getFaceCons, clFindDT and getInternal on the oracle's initial state."""
     import oracle as O
     from pyugks_amd import lib as L
     print("[gpu] pyopencl call form", flush=True)
@@ -153,6 +153,8 @@ def test_reference_call_form_drives_the_kernels(hip):
               np.int32(0))
     prg.getFaceCons(None, gsz, None, face_f, dev["normal"], np.int32(0), prim, np.int32(1), np.int32(1))
     prg.clFindDT(None, gsz, None, dev["xy"], dev["area"], macro_D, ts, np.float64(gp.Kn_eff))
+    txyz_D, stress_D = L.DeviceArray((ni, nj, 4), np.float64, fill=0), L.DeviceArray((ni, nj, 4), np.float64, fill=0)
+    prg.getInternal(None, gsz, None, dev["f"], txyz_D, stress_D)
     L.synchronize()
     # the oracle's own kernels on the same inputs
     lib = O.lib()
@@ -177,4 +179,6 @@ def test_reference_call_form_drives_the_kernels(hip):
     assert rel_err(face_f.download()[inner], of[inner]) <= TOL
     assert rel_err(prim.download()[G + 1:G + ni, G + 1:G + nj], opr[G + 1:G + ni, G + 1:G + nj]) <= TOL
     assert rel_err(ts.download(), ots) <= TOL
+    otxyz, ostress = b.internal()
+    assert rel_err(txyz_D.download(), otxyz) <= TOL and rel_err(stress_D.download(), ostress) <= TOL
     prg.close()
