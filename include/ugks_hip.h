@@ -307,6 +307,26 @@ int ugks_copy_kernel(void *stream, const void *src, void *dst, int64_t bytes);
 int ugks_f64_to_f32(void *stream, const void *src, void *dst, int64_t n);
 int ugks_f32_to_f64(void *stream, const void *src, void *dst, int64_t n);
 
+/* ---- allocation-free step ------------------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): nsteps fused steps of ONE case of up to 16 blocks, the
+ * same launches as ugks_step_plan_create + ugks_step_run, with every piece of
+ * working memory (face sums, face states and records, cell moments, the block
+ * records and work tables, the time-step state) carved from a library-owned
+ * static device array of ugks_step_noalloc_capacity() = 64 MiB per GPU, which
+ * each GPU holds until the process ends.  dt is a ugks_dt_desc in DEVICE
+ * memory, read at the start and written back at the end (no host transfer or
+ * synchronisation).  Blocks and buffers are the caller's, as for a plan.
+ * Returns -2, launching nothing, when the case needs more working memory
+ * (about 452 bytes per interior face plus 32 per cell, ghosts included).
+ * Calls on one GPU share its array and keep the last call's contents; calls
+ * that overlap on one GPU race.  The per-call set-up re-derives everything
+ * it reads, so consecutive calls may advance different cases. */
+size_t ugks_step_noalloc_capacity(void);
+int ugks_step_run_noalloc(ugks_block *const *blocks, const ugks_block_buffers *bufs, const ugks_bc_desc *bcs,
+                          int32_t nblocks, int32_t precision, ugks_dt_desc *dt, int32_t nsteps, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

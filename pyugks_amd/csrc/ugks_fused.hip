@@ -2181,3 +2181,207 @@ int ugks_f32_to_f64(void *stream, const void *src, void *dst, int64_t n) {
 
 }  // extern "C"
 
+// ================================================== allocation-free step ==
+// ugks_step_run_noalloc: the plan's step without a plan, for callers that
+// cannot allocate device memory.  Its working memory is carved from one
+// file-scope device array per GPU; the host refers to that array only through
+// the address hipGetSymbolAddress returns for the current device.
+namespace {
+constexpr size_t kNoAllocBytes = (size_t)64 << 20;   // working memory per GPU
+constexpr int kNoAllocMaxBlocks = 16;                 // blocks of the one case a call advances
+constexpr int kNoAllocMaxDevices = 64;
+__device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+unsigned char *g_scratch[kNoAllocMaxDevices];         // the array's address on each device
+
+// the small tables of a call, carried to the device as one kernel argument
+struct NoAllocTables {
+    long long off[4][kNoAllocMaxBlocks + 1];          // face, cell, all-cell and tile prefix offsets
+    WallItem walls[4 * kNoAllocMaxBlocks];
+    GhostItem ghosts[4 * kNoAllocMaxBlocks];
+    int udf[kNoAllocMaxBlocks];
+    int nb, nwall, nghost, nudf;
+};
+static_assert(sizeof(NoAllocTables) < 4000, "the tables travel as a kernel argument");
+
+__global__ void k_put_block(FBlock F, FBlock *dst) {
+    if (threadIdx.x == 0) *dst = F;
+}
+__global__ void k_put_tables(NoAllocTables T, long long *off, WallItem *walls, GhostItem *ghosts, int *udf,
+                             int *case_first) {
+    const int t = threadIdx.x;
+    if (t < T.nb + 1)
+        for (int k = 0; k < 4; k++) off[k * (T.nb + 1) + t] = T.off[k][t];
+    if (t < T.nwall) walls[t] = T.walls[t];
+    if (t < T.nghost) ghosts[t] = T.ghosts[t];
+    if (t < T.nudf) udf[t] = T.udf[t];
+    if (t == 0) { case_first[0] = 0; case_first[1] = T.nb; }
+}
+// the case's time-step state from the caller's descriptor (device memory), and
+// the per-block time-step words cleared (the last call left its own there)
+__global__ void k_dt_in(const ugks_dt_desc *in, DtState *ds, double *max_dt, unsigned long long *maxbits, int nb) {
+    const int t = threadIdx.x;
+    if (t < nb) { max_dt[t] = in->max_dt; maxbits[t] = 0ull; }
+    if (t == 0) {
+        DtState d{in->dt, in->dt, in->dt, in->time, in->CFL, in->delta_dt, in->suggest_dt, in->Kn_eff,
+                  in->step, 0, in->force_time, 0};
+        flux_loop_dts(d, in->max_dt);
+        *ds = d;
+    }
+}
+__global__ void k_dt_out(const DtState *ds, const double *max_dt, int nb, ugks_dt_desc *out) {
+    if (threadIdx.x != 0) return;
+    const DtState d = *ds;
+    double M = 0.0;
+    for (int b = 0; b < nb; b++) M = (b == 0 || max_dt[b] < M) ? max_dt[b] : M;
+    ugks_dt_desc o;
+    o.dt = d.dt; o.time = d.time; o.CFL = d.CFL; o.delta_dt = d.delta_dt; o.suggest_dt = d.suggest_dt;
+    o.Kn_eff = d.Kn; o.step = d.step; o.overshoot = d.overshoot; o.force_time = d.force_time; o.pad = 0;
+    o.max_dt = M;
+    *out = o;
+}
+
+// the array's address on the current device, looked up once per device
+int scratch_base(unsigned char **base) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return ugks_rt::hip_fail(e, "step noalloc: hipGetDevice");
+    if (dev < 0 || dev >= kNoAllocMaxDevices) return ugks_rt::fail(-1, "step noalloc: device index out of range");
+    if (!g_scratch[dev]) {
+        void *ptr = nullptr;
+        e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) return ugks_rt::hip_fail(e, "step noalloc: hipGetSymbolAddress(scratch)");
+        g_scratch[dev] = (unsigned char *)ptr;
+    }
+    *base = g_scratch[dev];
+    return 0;
+}
+}  // namespace
+
+extern "C" {
+
+size_t ugks_step_noalloc_capacity(void) { return kNoAllocBytes; }
+
+int ugks_step_run_noalloc(ugks_block *const *blocks, const ugks_block_buffers *bufs, const ugks_bc_desc *bcs,
+                          int32_t nblocks, int32_t precision, ugks_dt_desc *dt, int32_t nsteps, void *stream) {
+    if (!blocks || !bufs || !bcs || !dt || nsteps < 0 || nblocks < 1)
+        return ugks_rt::fail(-1, "step noalloc: bad arguments");
+    if (nblocks > kNoAllocMaxBlocks)
+        return ugks_rt::fail(-2, "step noalloc: more than 16 blocks in the case");
+    if (precision != UGKS_FP64 && precision != UGKS_MIXED) return ugks_rt::fail(-1, "step noalloc: bad precision");
+    hipStream_t s = (hipStream_t)stream;
+    ugks_step_plan P{};                    // on the stack: every device pointer below points into scratch
+    P.nb = nblocks;
+    P.precision = precision;
+    P.ncases = 1;
+    P.fa_waves = P.fc_waves = 4;
+    P.method = blocks[0] ? blocks[0]->c.method : -1;
+    FBlock host[kNoAllocMaxBlocks];
+    NoAllocTables T;
+    memset(&T, 0, sizeof T);
+    T.nb = nblocks;
+    int maxnv = 0, udf_wl = 0;
+    long long nw = 0, ng = 0;
+    for (int b = 0; b < nblocks; b++) {
+        if (!blocks[b]) return ugks_rt::fail(-1, "step noalloc: null block");
+        const Consts &C = blocks[b]->c;
+        if (C.method != P.method) P.method = -1;
+        bool wallf[4];
+        if (int rc = fill_block(host[b], blocks[b], bufs[b], bcs + 4 * b, 0, nblocks, nullptr, precision, wallf))
+            return rc;
+        FBlock &F = host[b];
+        T.off[0][b + 1] = T.off[0][b] + F.nS + (long long)F.nWi * C.nj;
+        T.off[1][b + 1] = T.off[1][b] + (long long)C.ni * C.nj;
+        T.off[2][b + 1] = T.off[2][b] + (long long)C.NI * C.NJ;
+        T.off[3][b + 1] = T.off[3][b] + (long long)((C.ni + 1) / 2) * ((C.nj + 1) / 2);
+        for (int k = 0; k < 4; k++) {
+            const int len = (k == GNORTH || k == GSOUTH) ? C.ni : C.nj;
+            if (wallf[k]) { T.walls[T.nwall++] = WallItem{b, k, nw}; nw += len; }
+            T.ghosts[T.nghost++] = GhostItem{b, k, ng};
+            ng += (long long)len * C.G;
+        }
+        if (C.time_udf) { T.udf[T.nudf++] = b; udf_wl = std::max(udf_wl, std::max(C.ni, C.nj)); }
+        if (C.NV > maxnv) maxnv = C.NV;
+    }
+    for (int b = 0; b < nblocks; b++)
+        for (int k = 0; k < 4; k++) {
+            const FBlock &F = host[b];
+            if ((F.bc[k] == 0 || F.bc[k] == 5) && (host[F.other[k]].c.NV != F.c.NV || host[F.other[k]].c.G != F.c.G))
+                return ugks_rt::fail(-1, "step noalloc: joined blocks differ in NV or ghost width");
+        }
+    int npt = (maxnv + kT - 1) / kT;
+    if (npt > 10) return ugks_rt::fail(-2, "step noalloc: NV above 2560 is not supported by the fused step");
+    if (npt == 9) npt = 10;
+    P.npt = npt;
+    P.nfaces = T.off[0][nblocks];
+    P.ncells = T.off[1][nblocks];
+    P.nall = T.off[2][nblocks];
+    P.ntiles = T.off[3][nblocks];
+    P.nwalls = nw;
+    P.nghosts = ng;
+    P.nwall_items = T.nwall;
+    P.nghost_items = T.nghost;
+    P.nudf = T.nudf;
+    P.udf_wl = udf_wl;
+    // carve the working memory: 256-byte aligned regions, refused if they do not fit
+    size_t at = 0;
+    bool fits = true;
+    auto carve = [&](size_t bytes) {
+        const size_t here = at;
+        at = (at + bytes + 255) & ~(size_t)255;
+        if (at > kNoAllocBytes) fits = false;
+        return here;
+    };
+    const size_t o_blocks = carve(sizeof(FBlock) * nblocks), o_off = carve(sizeof(long long) * 4 * (nblocks + 1)),
+                 o_walls = carve(sizeof(WallItem) * 4 * nblocks), o_ghosts = carve(sizeof(GhostItem) * 4 * nblocks),
+                 o_udf = carve(sizeof(int) * nblocks), o_case = carve(sizeof(int) * 2), o_state = carve(sizeof(DtState)),
+                 o_maxbits = carve(sizeof(unsigned long long) * nblocks), o_maxdt = carve(sizeof(double) * nblocks),
+                 o_cmom = carve(sizeof(d4) * (size_t)P.nall), o_fsum = carve(sizeof(double) * kFS * (size_t)P.nfaces),
+                 o_fstate = carve(sizeof(double) * kFT * (size_t)P.nfaces),
+                 o_frec = carve(sizeof(FaceRec) * (size_t)P.nfaces), o_tmap = carve(sizeof(int) * (size_t)P.nfaces);
+    if (!fits) {
+        char msg[200];
+        snprintf(msg, sizeof msg, "step noalloc: the case needs %zu bytes of working memory; the allocation-free "
+                                  "step has %zu per GPU", at, kNoAllocBytes);
+        return ugks_rt::fail(-2, msg);
+    }
+    unsigned char *base = nullptr;
+    if (int rc = scratch_base(&base)) return rc;
+    P.d_blocks = (FBlock *)(base + o_blocks);
+    long long *off = (long long *)(base + o_off);
+    P.d_face_off = off;
+    P.d_cell_off = off + (nblocks + 1);
+    P.d_all_off = off + 2 * (nblocks + 1);
+    P.d_tile_off = off + 3 * (nblocks + 1);
+    P.d_walls = (WallItem *)(base + o_walls);
+    P.d_ghosts = (GhostItem *)(base + o_ghosts);
+    P.d_udf_blocks = (int *)(base + o_udf);
+    P.d_case_first = (int *)(base + o_case);
+    P.d_state = (DtState *)(base + o_state);
+    P.d_maxbits = (unsigned long long *)(base + o_maxbits);
+    P.d_max_dt = (double *)(base + o_maxdt);
+    P.d_cmom = (d4 *)(base + o_cmom);
+    P.d_fsum = (double *)(base + o_fsum);
+    P.d_fstate = (double *)(base + o_fstate);
+    P.d_frec = (FaceRec *)(base + o_frec);
+    P.d_tmap = (int *)(base + o_tmap);
+    for (int b = 0; b < nblocks; b++) {
+        host[b].cmom = P.d_cmom + T.off[2][b];
+        hipLaunchKernelGGL(k_put_block, dim3(1), dim3(64), 0, s, host[b], P.d_blocks + b);
+    }
+    hipLaunchKernelGGL(k_put_tables, dim3(1), dim3(kT), 0, s, T, off, P.d_walls, P.d_ghosts, P.d_udf_blocks,
+                       P.d_case_first);
+    hipLaunchKernelGGL(k_dt_in, dim3(1), dim3(64), 0, s, (const ugks_dt_desc *)dt, P.d_state, P.d_max_dt, P.d_maxbits,
+                       nblocks);
+    int rc = ugks_rt::check_launch("step noalloc: set-up");
+    if (!rc) rc = refresh(&P, s);
+    for (int k = 0; k < nsteps && !rc; k++)
+        rc = precision == UGKS_FP64 ? dispatch_npt<double>(&P, s, nullptr) : dispatch_npt<float>(&P, s, nullptr);
+    if (!rc) {
+        hipLaunchKernelGGL(k_dt_out, dim3(1), dim3(64), 0, s, (const DtState *)P.d_state, (const double *)P.d_max_dt,
+                           nblocks, dt);
+        rc = ugks_rt::check_launch("step noalloc: time-step state");
+    }
+    return rc;
+}
+
+}  // extern "C"

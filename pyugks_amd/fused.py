@@ -166,3 +166,28 @@ class StepPlan:
         except Exception:
             pass
 
+
+def run_noalloc(sim, precision: str, nsteps: int, dt_dev=None, stream=None):
+    """Advance one case by nsteps through ugks_step_run_noalloc, the step that
+    allocates no device memory (its working memory is the library's static
+    per-GPU array).  The case's buffers and, for the mixed step, the FP32
+    copies of f are prepared beforehand (here, outside the call); ``dt_dev``
+    is a device ugks_dt_desc, allocated here when not given.  Returns it; the
+    simulation's dt, time and step are updated from it."""
+    if precision not in ("fp64", "mixed"):
+        raise ValueError("precision must be 'fp64' or 'mixed'")
+    lib = L.load()
+    blocks, bufs, bcs, handles, case_of, dts = plan_inputs([sim], precision, stream)
+    if dt_dev is None:
+        dt_dev = L.DeviceArray((C.sizeof(L.DtDesc),), np.uint8)
+    dt_dev.upload(np.frombuffer(bytes(dts[0]), dtype=np.uint8), stream)
+    L.check(lib.ugks_step_run_noalloc(handles, bufs, bcs, len(blocks), L.UGKS_MIXED if precision == "mixed" else
+                                      L.UGKS_FP64, dt_dev.ptr, int(nsteps), stream), "ugks_step_run_noalloc")
+    d = L.DtDesc.from_buffer_copy(dt_dev.download(stream).tobytes())
+    g = sim.gdata
+    g.dt, g.time, g.step = d.dt, d.time, d.step
+    sim.step = d.step
+    if precision == "mixed":
+        for _, b in blocks:
+            L.check(lib.ugks_f32_to_f64(stream, b.f32_D.ptr, b.f_D.ptr, int(np.prod(b.f32_D.shape))), "f32_to_f64")
+    return d

@@ -80,6 +80,7 @@ FUSED = {
     "ugks_copy_kernel": [P, P, P, C.c_int64],
     "ugks_f64_to_f32": [P, P, P, C.c_int64],
     "ugks_f32_to_f64": [P, P, P, C.c_int64],
+    "ugks_step_run_noalloc": [P, P, P, I, I, P, I, P],
 }
 UGKS_FP64, UGKS_MIXED = 0, 1
 
@@ -175,6 +176,8 @@ def load(path: str | None = None):
         f.restype = C.c_int
     lib.ugks_last_error.restype = C.c_char_p
     lib.ugks_last_error.argtypes = []
+    lib.ugks_step_noalloc_capacity.restype = C.c_size_t
+    lib.ugks_step_noalloc_capacity.argtypes = []
     _lib = lib
     return lib
 
@@ -188,7 +191,8 @@ def check(rc: int, what: str = ""):
 
 def exported_symbols():
     """Every C symbol include/ugks_hip.h declares."""
-    return (list(RUNTIME) + ["ugks_last_error"] + ["ugks_" + k for k in KERNELS] + list(FUSED))
+    return (list(RUNTIME) + ["ugks_last_error", "ugks_step_noalloc_capacity"] + ["ugks_" + k for k in KERNELS]
+            + list(FUSED))
 
 
 # ------------------------------------------------------------------ memory --
