@@ -172,6 +172,16 @@ def test_first_step_all_buffers(hip, name):
     bad = {k: v for k, v in worst.items() if not v <= TOL}
     print("worst", max(worst.values()))
     assert not bad, bad
+    if "rho" in name:
+        # with a 1% density perturbation every slope is real, so the slope
+        # buffers are also held to their own maximum, with no floor.  aL, aR
+        # and aT are micro slopes of differences of nearly equal moment sums
+        # over all velocities, whose rounding that maximum does not bound as
+        # tightly: they get 2e-12 (1.06e-12 measured with 41^2 velocities)
+        for bi, (bg, bo) in enumerate(zip(gpu.blocks, ora.blocks)):
+            for gname, tol in (("sigma_D", TOL), ("aL_D", 2 * TOL), ("aR_D", 2 * TOL), ("aT_D", 2 * TOL)):
+                e = rel_err(getattr(bg, gname).download(), bo[BUFFERS[gname]])
+                assert e <= tol, (name, bi, gname, e)
 
 
 @pytest.mark.parametrize("name", list(CASES))
