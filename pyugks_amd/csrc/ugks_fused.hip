@@ -433,11 +433,19 @@ template <typename T, int M>
 __device__ __forceinline__ void stencil_dma(const Consts &C, const Stencil<T> &st, unsigned vb, AS3 char *lb) {
     constexpr int sz = 16;   // one (g, h) pair of doubles per lane
     const int ncell = method_of<M>(C) == 1 ? 6 : 4;
+    // The row addresses are formed again at every call, by scalar adds.
+    // Hoisted out of the velocity loop, the six 64-bit global and six LDS
+    // row addresses took 18 scalar registers the flux pass does not have:
+    // they were spilled to vector lanes and read back with v_readlane
+    // (vector instructions, each followed by wait states) in every iteration.
+    unsigned sb = st.sb, lds = (unsigned)(size_t)lb;
+    __asm__ volatile("" : "+s"(sb), "+s"(lds));
 #pragma unroll
     for (int r = 0; r < 6; r++) {
         if (r < ncell) {
-            const AS1 char *src = st.base + r * st.sb;
-            __builtin_amdgcn_global_load_lds((const AS1 void *)(src + vb), (AS3 void *)(lb + r * kStenRow<T>), sz, 0, 0);
+            const AS1 char *src = st.base + r * sb;
+            __builtin_amdgcn_global_load_lds((const AS1 void *)(src + vb), (AS3 void *)(size_t)(lds + r * kStenRow<T>), sz,
+                                             0, 0);
         }
     }
 }
