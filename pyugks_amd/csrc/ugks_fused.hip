@@ -37,6 +37,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <vector>
 
 #include "../../include/ugks_hip.h"
@@ -2208,7 +2209,7 @@ constexpr size_t kNoAllocBytes = (size_t)64 << 20;   // working memory per GPU
 constexpr int kNoAllocMaxBlocks = 16;                 // blocks of the one case a call advances
 constexpr int kNoAllocMaxDevices = 64;
 __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
-unsigned char *g_scratch[kNoAllocMaxDevices];         // the array's address on each device
+std::atomic<unsigned char *> g_scratch[kNoAllocMaxDevices];   // the array's address on each device
 
 // the small tables of a call, carried to the device as one kernel argument
 struct NoAllocTables {
@@ -2263,13 +2264,16 @@ int scratch_base(unsigned char **base) {
     hipError_t e = hipGetDevice(&dev);
     if (e != hipSuccess) return ugks_rt::hip_fail(e, "step noalloc: hipGetDevice");
     if (dev < 0 || dev >= kNoAllocMaxDevices) return ugks_rt::fail(-1, "step noalloc: device index out of range");
-    if (!g_scratch[dev]) {
+    unsigned char *known = g_scratch[dev].load(std::memory_order_acquire);
+    if (!known) {
+        // threads that race here look up the same address; either store is right
         void *ptr = nullptr;
         e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
         if (e != hipSuccess || !ptr) return ugks_rt::hip_fail(e, "step noalloc: hipGetSymbolAddress(scratch)");
-        g_scratch[dev] = (unsigned char *)ptr;
+        known = (unsigned char *)ptr;
+        g_scratch[dev].store(known, std::memory_order_release);
     }
-    *base = g_scratch[dev];
+    *base = known;
     return 0;
 }
 }  // namespace
@@ -2328,6 +2332,10 @@ int ugks_step_run_noalloc(ugks_block *const *blocks, const ugks_block_buffers *b
     int npt = (maxnv + kT - 1) / kT;
     if (npt > 10) return ugks_rt::fail(-2, "step noalloc: NV above 2560 is not supported by the fused step");
     if (npt == 9) npt = 10;
+    for (int b = 0; b < nblocks; b++)
+        if (host[b].c.nut > 64 || host[b].c.nvt > 64)
+            return ugks_rt::fail(-2, "step noalloc: more than 64 velocities per axis");
+    if (T.off[0][nblocks] >= (1LL << 31)) return ugks_rt::fail(-2, "step noalloc: 2^31 or more interior faces");
     P.npt = npt;
     P.nfaces = T.off[0][nblocks];
     P.ncells = T.off[1][nblocks];
