@@ -877,8 +877,15 @@ __global__ void __launch_bounds__(64 * FW) __attribute__((amdgpu_waves_per_eu(kF
 // ============================================================== k_wall ====
 struct WallItem { int block, wall; long long start; };
 
+// Three waves per SIMD asked of the wall pass up to 7 velocities per thread
+// (NV <= 1792): the compiler's own choice, 170-184 VGPRs, leaves two, and the
+// pass, a chain of loads and workgroup reductions per wall face, then waits
+// with nothing to switch to.  Measured on the 64-case batch (41^2 velocities
+// at most): 1.10 -> 0.86 ms mixed, 1.20 -> 0.97 ms FP64 (36 bytes spilled).
+// Beyond 7 the spills grow and the pass is a negligible share of the step.
+template <int NPT> constexpr int kWallWaves = NPT <= 7 ? 3 : 1;
 template <typename T, int NPT>
-__global__ void __launch_bounds__(kT) k_wall(const FBlock *__restrict__ blocks, const WallItem *__restrict__ items,
+__global__ void __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(kWallWaves<NPT>))) k_wall(const FBlock *__restrict__ blocks, const WallItem *__restrict__ items,
                                              int nitems, const DtState *__restrict__ ds) {
     __shared__ double red[kWavesT][16];
     __shared__ P2<T> wd[NPT * kT];   // reflected distribution (clAdsorb.c:377) for the specular mirror lookup
