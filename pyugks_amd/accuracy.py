@@ -53,6 +53,8 @@ def _run(g, precision, checkpoints):
         if flag != 0.0:
             break
     plan.close()
+    for b in sim.blocks:
+        b.free_device()        # tens of GB at these sizes: released now, not at garbage collection
     return out
 
 

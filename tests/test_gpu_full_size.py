@@ -12,6 +12,7 @@
   cases of the batch on their own time steps; no error flag.
 """
 import copy
+import gc
 import os
 
 import numpy as np
@@ -43,6 +44,13 @@ def _run(gs, precision, nsteps):
            "drift": [abs(sum(s.total_mass()) - a) / a for s, a in zip(sims, m0)],
            "dt": [d.dt for d in states], "flag": plan.flag()}
     plan.close()
+    # the simulations hold tens of GB of device memory in reference cycles:
+    # release it now, not whenever the cyclic collector next runs
+    for sim in sims:
+        for b in sim.blocks:
+            b.free_device()
+    del plan, sims
+    gc.collect()
     return out
 
 
@@ -65,6 +73,7 @@ def test_cavity_64_gh24_vanleer_fp64_and_mixed_against_the_oracle(hip):
         assert rel_err(bg.f_D.download(), bo["f"]) <= TOL, step
     assert rel_err(gpu.blocks[0].Q_D.download(), ora.blocks[0]["Q"], SCALE_FLOOR["Q"]) <= TOL
     plan.close()
+    gpu.blocks[0].free_device()
     mixed = _run([g], "mixed", 20)
     e = field_errors(mixed["macro"][0], [ora.blocks[0]["macro"]])
     print(f"[gpu] mixed vs oracle after 20 steps: {e['rho_linf']:.2e} {e['U_linf']:.2e} {e['T_linf']:.2e}",
@@ -120,3 +129,21 @@ def test_adsorbing_channel_mixed_tracks_fp64(hip):
         assert e[k] <= 1e-5, (k, e[k])
     # the open channel's mass changes (inflow, outflow, adsorption), the same in both
     assert abs(mix["drift"][0] - ref["drift"][0]) <= 1e-5 * max(ref["drift"][0], 1e-12) + 1e-9
+
+
+def test_accuracy_driver_reports_each_checkpoint(hip, capsys):
+    """python -m pyugks_amd.accuracy: one JSON line per checkpoint with the
+    mixed-vs-FP64 errors and both mass drifts (two short checkpoints of the
+    channel here)."""
+    import json
+
+    from pyugks_amd import accuracy
+    print("[gpu] accuracy driver, channel, 2 checkpoints", flush=True)
+    capsys.readouterr()
+    accuracy.main(["--cases", "channel", "--steps", "1,2", "--cfl", "0.4"])
+    lines = [json.loads(x) for x in capsys.readouterr().out.splitlines() if x.startswith("{")]
+    assert [d["step"] for d in lines] == [1, 2]
+    for d in lines:
+        assert d["case"] == "channel" and d["flag_fp64"] == 0.0 and d["flag_mixed"] == 0.0
+        assert d["U_linf"] <= 1e-5 and d["rho_linf"] <= 1e-5 and d["T_linf"] <= 1e-5
+        assert abs(d["drift_mixed"] - d["drift_fp64"]) <= 1e-9
