@@ -58,3 +58,30 @@ def test_noalloc_refuses_a_case_beyond_its_working_memory(hip):
     with pytest.raises(L.UGKSError, match="working memory"):
         run_noalloc(sim, "fp64", 1)
     assert np.array_equal(sim.blocks[0].f(), f0)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_noalloc_step_with_time_dependent_walls_and_joins(hip, precision):
+    """A case whose lid speed depends on time (a wall-value expression the
+    step re-evaluates every step) over two joined blocks, on an adsorbing
+    channel's walls: the allocation-free step equals the plan's, bit for bit."""
+    from pyugks_amd import bc as bcd
+    from pyugks_amd.fused import StepPlan, run_noalloc
+    from pyugks_amd.sim import UGKSim
+    g = cases.cavity(10, ("NC", 8), "WENO5", 0.05, lid=0.15, n_cells_j=8, nb=(2, 1))
+    for blk in g.blocks:
+        blk.bc_list[bcd.NORTH] = bcd.DiffuseBC(D=1.0, T=1.0, U=0.15, UDF_U="0.2*sin(2*PI*t/0.05) + 0.05*s")
+    ch = cases.cfg5_channel(widths=[6, 8], height=6, nc=8, Kn=0.05)
+    for gg, nsteps in ((g, 3), (ch, 2)):
+        print(f"[gpu] noalloc {precision}: {len(gg.blocks)} blocks, {nsteps} steps", flush=True)
+        a, b = UGKSim(copy.deepcopy(gg)), UGKSim(copy.deepcopy(gg))
+        plan = StepPlan(a, precision)
+        plan.run(nsteps)
+        d = plan.state()
+        run_noalloc(b, precision, nsteps)
+        assert (b.gdata.dt, b.gdata.time, b.step) == (d.dt, d.time, d.step)
+        _same(a, b, plan if precision == "mixed" else None)
+        for x, y in zip(a.blocks, b.blocks):
+            assert np.array_equal(x.wall_prop_D.download(), y.wall_prop_D.download())
+            assert np.array_equal(x.wall_cover_D.download(), y.wall_cover_D.download())
+        plan.close()
