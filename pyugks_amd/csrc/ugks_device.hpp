@@ -340,22 +340,63 @@ __device__ __forceinline__ d2 shakhov_eq(const Consts &C, d4 p, d2 Q, d2 uv, int
 __device__ __forceinline__ double van_leer(double s1, double s2) {
     return (ocl_sign(s1) + ocl_sign(s2)) * (fabs(s1) * fabs(s2)) / (fabs(s1) + fabs(s2) + 1e-16);
 }
-// WENO5 slope (clFunctions.c:268-312), one component; s0..s4 upwind-ordered
-__device__ __forceinline__ double weno5(double s0, double s1, double s2, double s3, double s4, double dx) {
-    double t1 = s0 - 2.0 * s1 + s2, t2 = s0 - 4.0 * s1 + 3.0 * s2;
-    double b0 = (13.0 / 12.0) * t1 * t1 + 0.25 * t2 * t2;
-    t1 = s1 - 2.0 * s2 + s3; t2 = s1 - s3;
-    double b1 = (13.0 / 12.0) * t1 * t1 + 0.25 * t2 * t2;
-    t1 = s2 - 2.0 * s3 + s4; t2 = 3.0 * s2 - 4.0 * s3 + s4;
-    double b2 = (13.0 / 12.0) * t1 * t1 + 0.25 * t2 * t2;
-    double e0 = 1E-16 + b0, e1 = 1E-16 + b1, e2 = 1E-16 + b2;
-    double a0 = 0.1 / (e0 * e0), a1 = 0.6 / (e1 * e1), a2 = 0.3 / (e2 * e2);
-    double sum = a0 + a1 + a2;
-    double w0 = a0 / sum, w1 = a1 / sum, w2 = a2 / sum;
-    double f0 = (1. / 3.) * s0 - (7. / 6.) * s1 + (11. / 6.) * s2;
-    double f1 = -(1. / 6.) * s1 + (5. / 6.) * s2 + (1. / 3.) * s3;
-    double f2 = (1. / 3.) * s2 + (5. / 6.) * s3 - (1. / 6.) * s4;
-    return ((w0 * f0 + w1 * f1 + w2 * f2) - s2) / dx;
+// WENO5 slope of both components (g, h) (clFunctions.c:268-312), written
+// in the first differences d_k = s_k - s_{k-1}:
+//   smoothness  12 b_0 = 13 (d2 - d1)^2 + 3 (3 d2 - d1)^2
+//               12 b_1 = 13 (d3 - d2)^2 + 3 (d2 + d3)^2
+//               12 b_2 = 13 (d4 - d3)^2 + 3 (3 d3 - d4)^2
+//   candidates  f_0 - s2 = -d1/3 + 5 d2/6,  f_1 - s2 = d2/6 + d3/3,
+//               f_2 - s2 = 2 d3/3 - d4/6
+// with eps scaled by 12 alongside (the weights only see the ratios), so that
+// (f_L - s2) is formed without cancelling against s2.  The nonlinear weights
+// a_k = d_k / (eps + b_k)^2 are written 1 / e_k^2 with e_k = 12 (eps + b_k)
+// / sqrt(d_k): the linear weights d = (0.1, 0.6, 0.3) ride in the constants
+// of e_k, which is one multiply and two FMAs from the differences.  They
+// enter only through sum(a_k (f_k - s2)) / sum(a_k): in double both sums are
+// scaled by prod_k e_k^2, leaving one quotient per component, and the two
+// quotients share one reciprocal.  (In float that product can underflow:
+// the FP32 form takes reciprocal estimates of the weights instead.)
+// e_k = A_k u_k^2 + B_k v_k^2 + E_k with A_k = 13 / sqrt(d_k), B_k = 3 / sqrt(d_k), E_k = 1.2e-15 / sqrt(d_k)
+constexpr double kWA0 = 41.10960958218893, kWB0 = 9.486832980505138, kWE0 = 3.794733192202055e-15;
+constexpr double kWA1 = 16.782927833565473, kWB1 = 3.8729833462074166, kWE1 = 1.5491933384829667e-15;
+constexpr double kWA2 = 23.7346441585572, kWB2 = 5.477225575051661, kWE2 = 2.1908902300206646e-15;
+struct WenoParts { double num, den; };
+__device__ __forceinline__ WenoParts weno5_parts(double s0, double s1, double s2, double s3, double s4) {
+    const double d1 = s1 - s0, d2 = s2 - s1, d3 = s3 - s2, d4 = s4 - s3;
+    const double u0 = d2 - d1, v0 = 3.0 * d2 - d1;
+    const double u1 = d3 - d2, v1 = d2 + d3;
+    const double u2 = d4 - d3, v2 = 3.0 * d3 - d4;
+    const double e0 = fma(kWA0 * u0, u0, fma(kWB0 * v0, v0, kWE0));
+    const double e1 = fma(kWA1 * u1, u1, fma(kWB1 * v1, v1, kWE1));
+    const double e2 = fma(kWA2 * u2, u2, fma(kWB2 * v2, v2, kWE2));
+    const double q0 = e0 * e0, q1 = e1 * e1, q2 = e2 * e2;
+    const double p0 = q1 * q2, p1 = q0 * q2, p2 = q0 * q1;
+    // (f_k - s2)
+    const double c0 = (-1.0 / 3.0) * d1 + (5.0 / 6.0) * d2;
+    const double c1 = (1.0 / 6.0) * d2 + (1.0 / 3.0) * d3;
+    const double c2 = (2.0 / 3.0) * d3 - (1.0 / 6.0) * d4;
+    return WenoParts{p0 * c0 + p1 * c1 + p2 * c2, (p0 + p1) + p2};
+}
+// the two slopes (f_L - s2) / dist of a (g, h) stencil
+__device__ __forceinline__ void weno5_slope_pair(double g0, double g1, double g2, double g3, double g4, double h0,
+                                                 double h1, double h2, double h3, double h4, double rdx, double &sgx,
+                                                 double &sgy) {
+    const WenoParts a = weno5_parts(g0, g1, g2, g3, g4);
+    const WenoParts b = weno5_parts(h0, h1, h2, h3, h4);
+    // 1 / (den_a den_b) from the hardware estimate and two Newton steps; the
+    // dens are >= (1.2e-15)^4 and far below overflow, so is their product
+    const double D = a.den * b.den;
+    double r = __builtin_amdgcn_rcp(D);
+    double e = fma(-D, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-D, r, 1.0);
+    r = fma(r, e, r);
+    const double ra = r * b.den, rb = r * a.den;     // 1 / den_a, 1 / den_b
+    double qa = a.num * ra, qb = b.num * rb;
+    qa = fma(ra, fma(-a.den, qa, a.num), qa);        // one residual correction each
+    qb = fma(rb, fma(-b.den, qb, b.num), qb);
+    sgx = qa * rdx;
+    sgy = qb * rdx;
 }
 // minmod + NND (clFunctions.c:320-352), one component
 __device__ __forceinline__ double minmod(double a, double b) {

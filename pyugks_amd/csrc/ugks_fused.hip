@@ -154,62 +154,10 @@ template <int N> __device__ __forceinline__ void block_sum(double (&a)[N], doubl
 // ---- reconstruction (iFace, clUGKS.c:134-195) in precision T ---------------
 __device__ __forceinline__ float rcp_T(float x) { return __builtin_amdgcn_rcpf(x); }
 
-// WENO5 slope of both components (g, h) (clFunctions.c:268-312), written
-// in the first differences d_k = s_k - s_{k-1}:
-//   smoothness  12 b_0 = 13 (d2 - d1)^2 + 3 (3 d2 - d1)^2
-//               12 b_1 = 13 (d3 - d2)^2 + 3 (d2 + d3)^2
-//               12 b_2 = 13 (d4 - d3)^2 + 3 (3 d3 - d4)^2
-//   candidates  f_0 - s2 = -d1/3 + 5 d2/6,  f_1 - s2 = d2/6 + d3/3,
-//               f_2 - s2 = 2 d3/3 - d4/6
-// with eps scaled by 12 alongside (the weights only see the ratios), so that
-// (f_L - s2) is formed without cancelling against s2.  The nonlinear weights
-// a_k = d_k / (eps + b_k)^2 are written 1 / e_k^2 with e_k = 12 (eps + b_k)
-// / sqrt(d_k): the linear weights d = (0.1, 0.6, 0.3) ride in the constants
-// of e_k, which is one multiply and two FMAs from the differences.  They
-// enter only through sum(a_k (f_k - s2)) / sum(a_k): in double both sums are
-// scaled by prod_k e_k^2, leaving one quotient per component, and the two
-// quotients share one reciprocal.  In float that product can underflow, so
-// the weights use reciprocal estimates instead.
-// e_k = A_k u_k^2 + B_k v_k^2 + E_k with A_k = 13 / sqrt(d_k), B_k = 3 / sqrt(d_k), E_k = 1.2e-15 / sqrt(d_k)
-constexpr double kWA0 = 41.10960958218893, kWB0 = 9.486832980505138, kWE0 = 3.794733192202055e-15;
-constexpr double kWA1 = 16.782927833565473, kWB1 = 3.8729833462074166, kWE1 = 1.5491933384829667e-15;
-constexpr double kWA2 = 23.7346441585572, kWB2 = 5.477225575051661, kWE2 = 2.1908902300206646e-15;
-struct WenoParts { double num, den; };
-__device__ __forceinline__ WenoParts weno5_parts(double s0, double s1, double s2, double s3, double s4) {
-    const double d1 = s1 - s0, d2 = s2 - s1, d3 = s3 - s2, d4 = s4 - s3;
-    const double u0 = d2 - d1, v0 = 3.0 * d2 - d1;
-    const double u1 = d3 - d2, v1 = d2 + d3;
-    const double u2 = d4 - d3, v2 = 3.0 * d3 - d4;
-    const double e0 = fma(kWA0 * u0, u0, fma(kWB0 * v0, v0, kWE0));
-    const double e1 = fma(kWA1 * u1, u1, fma(kWB1 * v1, v1, kWE1));
-    const double e2 = fma(kWA2 * u2, u2, fma(kWB2 * v2, v2, kWE2));
-    const double q0 = e0 * e0, q1 = e1 * e1, q2 = e2 * e2;
-    const double p0 = q1 * q2, p1 = q0 * q2, p2 = q0 * q1;
-    // (f_k - s2)
-    const double c0 = (-1.0 / 3.0) * d1 + (5.0 / 6.0) * d2;
-    const double c1 = (1.0 / 6.0) * d2 + (1.0 / 3.0) * d3;
-    const double c2 = (2.0 / 3.0) * d3 - (1.0 / 6.0) * d4;
-    return WenoParts{p0 * c0 + p1 * c1 + p2 * c2, (p0 + p1) + p2};
-}
-// the two slopes (f_L - s2) / dist of a (g, h) stencil
+// WENO5 slopes of a (g, h) stencil: FP64 in weno5_slope_pair (ugks_device.hpp)
 __device__ __forceinline__ void weno5_pair(P2<double> s0, P2<double> s1, P2<double> s2, P2<double> s3, P2<double> s4,
                                            double rdx, double &sgx, double &sgy) {
-    const WenoParts a = weno5_parts(s0.x, s1.x, s2.x, s3.x, s4.x);
-    const WenoParts b = weno5_parts(s0.y, s1.y, s2.y, s3.y, s4.y);
-    // 1 / (den_a den_b) from the hardware estimate and two Newton steps; the
-    // dens are >= (1.2e-15)^4 and far below overflow, so is their product
-    const double D = a.den * b.den;
-    double r = __builtin_amdgcn_rcp(D);
-    double e = fma(-D, r, 1.0);
-    r = fma(r, e, r);
-    e = fma(-D, r, 1.0);
-    r = fma(r, e, r);
-    const double ra = r * b.den, rb = r * a.den;     // 1 / den_a, 1 / den_b
-    double qa = a.num * ra, qb = b.num * rb;
-    qa = fma(ra, fma(-a.den, qa, a.num), qa);        // one residual correction each
-    qb = fma(rb, fma(-b.den, qb, b.num), qb);
-    sgx = qa * rdx;
-    sgy = qb * rdx;
+    weno5_slope_pair(s0.x, s1.x, s2.x, s3.x, s4.x, s0.y, s1.y, s2.y, s3.y, s4.y, rdx, sgx, sgy);
 }
 __device__ __forceinline__ float weno5_f(float s0, float s1, float s2, float s3, float s4) {
     const float d1 = s1 - s0, d2 = s2 - s1, d3 = s3 - s2, d4 = s4 - s3;

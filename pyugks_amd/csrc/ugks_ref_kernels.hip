@@ -22,8 +22,19 @@ inline unsigned blocks_for_waves(long long nwaves) { return (unsigned)((nwaves +
 inline unsigned blocks_for_threads(long long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
 __device__ __forceinline__ long long wave_id() { return (long long)blockIdx.x * kWaves + (threadIdx.x >> 6); }
+// the same, known to the compiler to be uniform over the wavefront
+__device__ __forceinline__ long long wave_id_u() {
+    return (long long)blockIdx.x * kWaves + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+}
 __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
 __device__ __forceinline__ long long thread_id() { return (long long)blockIdx.x * kBlock + threadIdx.x; }
+
+// streaming (non-temporal) store of one (g, h) pair that nothing in the same
+// launch reads again
+__device__ __forceinline__ void stream_d2(d2 *p, d2 x) {
+    typedef double v2d __attribute__((ext_vector_type(2)));
+    __builtin_nontemporal_store(v2d{x.x, x.y}, reinterpret_cast<v2d *>(p));
+}
 
 // interior-face domain of the flux kernels (clUGKS.c:218-225, ugks_block.py:1105)
 struct FaceRange { int i0, ni, j0, nj; };
@@ -202,45 +213,73 @@ __global__ void k_getInternal(Consts C, const d2 *F, d4 *Txyz, d4 *Stress) {
 }
 
 // ---------------------------------------------------------------- faces
-// iFace (clUGKS.c:110-199): upwind reconstruction of f and its slope at a face
-__global__ void k_iFace(Consts C, const d2 *F, d2 *iface_f, d2 *fsigma, const d2 *centre, const d2 *mid_side,
-                        const d2 *normal, int face) {
-    long long w = wave_id();
-    int nvc = (C.NV + 63) / 64;
-    int ilim = face == SOUTH ? C.ni : C.ni + 1, jlim = face == SOUTH ? C.nj + 1 : C.nj;
-    if (w >= (long long)ilim * jlim * nvc) return;
-    int item = (int)(w / nvc), v = (int)(w % nvc) * 64 + lane_id();
-    if (v >= C.NV) return;
-    int gi = item / jlim + C.G, gj = item % jlim + C.G;
-    d2 n = normal[sidx(C, gi, gj, face)];
-    d2 uv = face_velocity(C, v, n);
-    int delta = upwind(uv.x);
-    const int SL = C.method == 1 ? 5 : 3, MID = C.method == 1 ? 2 : 1;
-    d2 fs[5], cs[5];
+// iFace (clUGKS.c:110-199): upwind reconstruction of f and its slope at a face.
+// One wavefront per face strides over the velocities; the face's geometry
+// (the stencil distances of both upwind choices) is formed once, and each
+// lane walks its own upwind stencil from a per-lane start and step.
+template <int METHOD>
+__device__ __forceinline__ void iface_face(const Consts &C, const d2 *__restrict__ F, d2 *__restrict__ iface_f,
+                                           d2 *__restrict__ fsigma, const d2 *__restrict__ centre, d2 mside, d2 n,
+                                           int gi, int gj, int face) {
+    constexpr int SL = METHOD == 1 ? 5 : 3, MID = METHOD == 1 ? 2 : 1;
+    // stencil cell si of upwind choice delta lies `off` cells along the face
+    // normal: delta = 1 at -MID-1+si, delta = 0 at MID-si
+    const long long step = face == SOUTH ? (long long)C.NV : (long long)C.NJ * C.NV;
+    const int di = face == SOUTH ? 0 : 1, dj = 1 - di;
+    double dist[2], dlen[2], rdist[2];
 #pragma unroll
-    for (int si = 0; si < 5; ++si) {
-        if (si >= SL) break;
-        int off = delta ? (-MID - 1 + si) : (MID - si);
-        int sx = face == SOUTH ? gi : gi + off, sy = face == SOUTH ? gj + off : gj;
-        fs[si] = F[fidx(C, sx, sy, v)];
-        cs[si] = centre[cidx(C, sx, sy)];
+    for (int delta = 0; delta < 2; ++delta) {
+        const int o0 = delta ? -MID - 1 : MID, os = delta ? 1 : -1;
+        const d2 c0 = centre[cidx(C, gi + di * o0, gj + dj * o0)];
+        const d2 c1 = centre[cidx(C, gi + di * (o0 + os), gj + dj * (o0 + os))];
+        const d2 cm = centre[cidx(C, gi + di * (o0 + MID * os), gj + dj * (o0 + MID * os))];
+        dist[delta] = length(mside - cm);
+        dlen[delta] = length(c1 - c0);
+        rdist[delta] = 1.0 / dist[delta];
     }
-    d2 mside = mid_side[sidx(C, gi, gj, face)];
-    double dist = length(mside - cs[MID]);
-    d2 sig;
-    if (C.method == 0) {
-        double d1 = length(cs[1] - cs[0]), d2_ = length(cs[0] - cs[1]);
-        sig.x = van_leer((fs[1].x - fs[0].x) / d1, (fs[2].x - fs[1].x) / d2_);
-        sig.y = van_leer((fs[1].y - fs[0].y) / d1, (fs[2].y - fs[1].y) / d2_);
-    } else if (C.method == 1) {
-        sig.x = weno5(fs[0].x, fs[1].x, fs[2].x, fs[3].x, fs[4].x, dist);
-        sig.y = weno5(fs[0].y, fs[1].y, fs[2].y, fs[3].y, fs[4].y, dist);
-    } else {
-        sig.x = nnd(fs[0].x, fs[1].x, fs[2].x, dist);
-        sig.y = nnd(fs[0].y, fs[1].y, fs[2].y, dist);
+    const d2 *fc = F + fidx(C, gi, gj, 0);
+    const size_t out0 = fidx(C, gi, gj, 0);
+    for (int v = lane_id(); v < C.NV; v += 64) {
+        const d2 uv = face_velocity(C, v, n);
+        const int delta = upwind(uv.x);
+        const long long o0 = delta ? -MID - 1 : MID, os = delta ? step : -step;
+        const d2 *p = fc + v + o0 * step;
+        d2 fs[SL];
+#pragma unroll
+        for (int si = 0; si < SL; ++si) fs[si] = p[si * os];
+        const double dst = delta ? dist[1] : dist[0];
+        d2 sig;
+        if constexpr (METHOD == 0) {
+            // clUGKS.c:179-180: both differences over the distance between cells 0 and 1
+            const double d = delta ? dlen[1] : dlen[0];
+            sig.x = van_leer((fs[1].x - fs[0].x) / d, (fs[2].x - fs[1].x) / d);
+            sig.y = van_leer((fs[1].y - fs[0].y) / d, (fs[2].y - fs[1].y) / d);
+        } else if constexpr (METHOD == 1) {
+            // the reference's WENO5 (clFunctions.c:268-312) rearranged to one
+            // quotient per component; it agrees with the literal form to rounding
+            weno5_slope_pair(fs[0].x, fs[1].x, fs[2].x, fs[3].x, fs[4].x, fs[0].y, fs[1].y, fs[2].y, fs[3].y, fs[4].y,
+                             delta ? rdist[1] : rdist[0], sig.x, sig.y);
+        } else {
+            sig.x = nnd(fs[0].x, fs[1].x, fs[2].x, dst);
+            sig.y = nnd(fs[0].y, fs[1].y, fs[2].y, dst);
+        }
+        stream_d2(iface_f + out0 + v, fs[MID] + sig * dst);
+        stream_d2(fsigma + out0 + v, sig);
     }
-    iface_f[fidx(C, gi, gj, v)] = fs[MID] + sig * dist;
-    fsigma[fidx(C, gi, gj, v)] = sig;
+}
+
+__global__ void __launch_bounds__(kBlock) k_iFace(Consts C, const d2 *__restrict__ F, d2 *__restrict__ iface_f,
+                                                  d2 *__restrict__ fsigma, const d2 *__restrict__ centre,
+                                                  const d2 *__restrict__ mid_side, const d2 *__restrict__ normal,
+                                                  int face) {
+    const long long w = wave_id_u();
+    const int ilim = face == SOUTH ? C.ni : C.ni + 1, jlim = face == SOUTH ? C.nj + 1 : C.nj;
+    if (w >= (long long)ilim * jlim) return;
+    const int gi = (int)(w / jlim) + C.G, gj = (int)(w % jlim) + C.G;
+    const d2 n = normal[sidx(C, gi, gj, face)], ms = mid_side[sidx(C, gi, gj, face)];
+    if (C.method == 0) iface_face<0>(C, F, iface_f, fsigma, centre, ms, n, gi, gj, face);
+    else if (C.method == 1) iface_face<1>(C, F, iface_f, fsigma, centre, ms, n, gi, gj, face);
+    else iface_face<2>(C, F, iface_f, fsigma, centre, ms, n, gi, gj, face);
 }
 
 // getFaceCons / getAL / getAR (clUGKS.c:209-399): sum psi f in face coordinates
@@ -356,25 +395,31 @@ __global__ void k_macroFlux(Consts C, const d2 *flux_f, const d2 *fsigma, d4 *fl
     }
 }
 
-// distFlux (clUGKS.c:683-772): per-velocity interface flux, in place
-__global__ void k_distFlux(Consts C, d2 *flux_f, const d2 *fsigma, const d2 *normal, const double *side_length,
-                           int face, double dt, const d4 *primary, const d4 *gaL, const d4 *gaR, const d4 *gaT,
-                           const double *gMxi, const d2 *faceQ, int ob, int ot) {
+// distFlux (clUGKS.c:683-772): per-velocity interface flux, in place.  One
+// wavefront per face: the relaxation time, time integrals and face state are
+// formed once, then the wavefront strides over the velocities.
+__global__ void __launch_bounds__(kBlock) k_distFlux(Consts C, d2 *__restrict__ flux_f, const d2 *__restrict__ fsigma,
+                                                     const d2 *__restrict__ normal,
+                                                     const double *__restrict__ side_length, int face, double dt,
+                                                     const d4 *__restrict__ primary, const d4 *__restrict__ gaL,
+                                                     const d4 *__restrict__ gaR, const d4 *__restrict__ gaT,
+                                                     const double *__restrict__ gMxi, const d2 *__restrict__ faceQ,
+                                                     int ob, int ot) {
     FaceRange r = face_range(C.ni, C.nj, face, ob, ot);
-    long long w = wave_id();
-    int nvc = (C.NV + 63) / 64;
-    if (w >= (long long)r.ni * r.nj * nvc) return;
-    int item = (int)(w / nvc), v = (int)(w % nvc) * 64 + lane_id();
-    if (v >= C.NV) return;
-    int gi = item / r.nj + r.i0 + C.G, gj = item % r.nj + r.j0 + C.G;
-    size_t k = cidx(C, gi, gj);
-    d4 prim = primary[k];
-    TimeInt mt = time_integrals(relax_time(C, prim), dt);
-    d2 n = normal[sidx(C, gi, gj, face)];
-    d2 uv = face_velocity(C, v, n);
-    size_t fi = fidx(C, gi, gj, v);
-    d2 ff = ugks_face_flux(C, prim, gaL[k], gaR[k], gaT[k], gMxi[3 * k + 2], mt, faceQ[k], uv, v, flux_f[fi], fsigma[fi]);
-    flux_f[fi] = side_length[sidx(C, gi, gj, face)] * ff;
+    const long long w = wave_id_u();
+    if (w >= (long long)r.ni * r.nj) return;
+    const int gi = (int)(w / r.nj) + r.i0 + C.G, gj = (int)(w % r.nj) + r.j0 + C.G;
+    const size_t k = cidx(C, gi, gj);
+    const d4 prim = primary[k], aL = gaL[k], aR = gaR[k], aT = gaT[k];
+    const TimeInt mt = time_integrals(relax_time(C, prim), dt);
+    const d2 n = normal[sidx(C, gi, gj, face)], Q = faceQ[k];
+    const double Mxi2 = gMxi[3 * k + 2], len = side_length[sidx(C, gi, gj, face)];
+    const size_t f0 = fidx(C, gi, gj, 0);
+    for (int v = lane_id(); v < C.NV; v += 64) {
+        const d2 uv = face_velocity(C, v, n);
+        const d2 ff = ugks_face_flux(C, prim, aL, aR, aT, Mxi2, mt, Q, uv, v, flux_f[f0 + v], fsigma[f0 + v]);
+        stream_d2(flux_f + f0 + v, len * ff);
+    }
 }
 
 // ---------------------------------------------------------------- walls
@@ -619,29 +664,34 @@ __global__ void k_updateMacro(Consts C, const d4 *fmS, const d4 *fmW, const doub
     residual[midx(C, mi, mj)] = old;
 }
 
-// UGKS_update (clUGKS.c:1189-1257)
-__global__ void k_UGKS_update(Consts C, d2 *F, const d2 *ffS, const d2 *ffW, const double *area, const d4 *macro,
-                              const d2 *gQ, const d4 *residual, double dt, double *flag) {
-    long long w = wave_id();
-    int nvc = (C.NV + 63) / 64;
-    if (w >= (long long)C.ni * C.nj * nvc) return;
-    int cell = (int)(w / nvc), v = (int)(w % nvc) * 64 + lane_id();
-    if (v >= C.NV) return;
-    int mi = cell / C.nj, mj = cell % C.nj, gi = mi + C.G, gj = mj + C.G;
-    d4 pold = residual[midx(C, mi, mj)], p = macro[midx(C, mi, mj)];
-    double tau_old = relax_time(C, pold), tau = relax_time(C, p);
-    d2 Q = gQ[midx(C, mi, mj)];
-    double A = area[cidx(C, gi, gj)];
-    d2 e = C.quad[v];
-    d2 feq_old = shakhov_eq(C, pold, Q, e, v);
-    d2 feq = shakhov_eq(C, p, Q, e, v);
-    size_t k = fidx(C, gi, gj, v);
-    d2 fo = F[k];
-    d2 fl = (((ffS[k] - ffS[fidx(C, gi, gj + 1, v)]) + ffW[k]) - ffW[fidx(C, gi + 1, gj, v)]) / A;
-    d2 relax = feq / tau + (feq_old - fo) / tau_old;
-    d2 fn = ((fo + fl) + (0.5 * dt) * relax) / (1.0 + 0.5 * dt / tau);
-    if (isnan(fn.x) || isnan(fn.y)) flag[0] = ERR_NAN;
-    F[k] = fn;
+// UGKS_update (clUGKS.c:1189-1257).  One wavefront per cell: the two
+// relaxation times and the cell's state are formed once, then the wavefront
+// strides over the velocities.
+__global__ void __launch_bounds__(kBlock) k_UGKS_update(Consts C, d2 *__restrict__ F, const d2 *__restrict__ ffS,
+                                                        const d2 *__restrict__ ffW, const double *__restrict__ area,
+                                                        const d4 *__restrict__ macro, const d2 *__restrict__ gQ,
+                                                        const d4 *__restrict__ residual, double dt, double *flag) {
+    const long long cell = wave_id_u();
+    if (cell >= (long long)C.ni * C.nj) return;
+    const int mi = (int)(cell / C.nj), mj = (int)(cell % C.nj), gi = mi + C.G, gj = mj + C.G;
+    const d4 pold = residual[midx(C, mi, mj)], p = macro[midx(C, mi, mj)];
+    const double tau_old = relax_time(C, pold), tau = relax_time(C, p);
+    const d2 Q = gQ[midx(C, mi, mj)];
+    const double A = area[cidx(C, gi, gj)], den = 1.0 + 0.5 * dt / tau;
+    const size_t k0 = fidx(C, gi, gj, 0), kN = fidx(C, gi, gj + 1, 0), kE = fidx(C, gi + 1, gj, 0);
+    bool bad = false;
+    for (int v = lane_id(); v < C.NV; v += 64) {
+        const d2 e = C.quad[v];
+        const d2 feq_old = shakhov_eq(C, pold, Q, e, v);
+        const d2 feq = shakhov_eq(C, p, Q, e, v);
+        const d2 fo = F[k0 + v];
+        const d2 fl = (((ffS[k0 + v] - ffS[kN + v]) + ffW[k0 + v]) - ffW[kE + v]) / A;
+        const d2 relax = feq / tau + (feq_old - fo) / tau_old;
+        const d2 fn = ((fo + fl) + (0.5 * dt) * relax) / den;
+        bad |= isnan(fn.x) || isnan(fn.y);
+        stream_d2(F + k0 + v, fn);
+    }
+    if (bad) flag[0] = ERR_NAN;
 }
 
 // getResidual (clUGKS.c:1259-1277)
@@ -916,7 +966,7 @@ int ugks_iFace(ugks_block *blk, void *stream, const void *Fin, void *iface_f, vo
     UGKS_FACE_CHECK(face);
     const Consts &C = blk->c;
     long long nf = face == SOUTH ? (long long)C.ni * (C.nj + 1) : (long long)(C.ni + 1) * C.nj;
-    hipLaunchKernelGGL(k_iFace, dim3(blocks_for_waves(nf * nvchunks(C))), dim3(kBlock), 0, S(stream), C,
+    hipLaunchKernelGGL(k_iFace, dim3(blocks_for_waves(nf)), dim3(kBlock), 0, S(stream), C,
                        (const d2 *)Fin, (d2 *)iface_f, (d2 *)fsigma, (const d2 *)centre, (const d2 *)mid_side,
                        (const d2 *)normal, (int)face);
     return ugks_rt::check_launch("iFace");
@@ -1022,7 +1072,7 @@ int ugks_distFlux(ugks_block *blk, void *stream, void *flux_f, const void *fsigm
     FaceRange r = face_range(C.ni, C.nj, face, offset_bottom, offset_top);
     long long n = (long long)r.ni * r.nj;
     if (n <= 0) return 0;
-    hipLaunchKernelGGL(k_distFlux, dim3(blocks_for_waves(n * nvchunks(C))), dim3(kBlock), 0, S(stream), C,
+    hipLaunchKernelGGL(k_distFlux, dim3(blocks_for_waves(n)), dim3(kBlock), 0, S(stream), C,
                        (d2 *)flux_f, (const d2 *)fsigma, (const d2 *)normal, (const double *)side_length, (int)face,
                        dt, (const d4 *)primary, (const d4 *)gaL, (const d4 *)gaR, (const d4 *)gaT,
                        (const double *)gMxi, (const d2 *)faceQ, (int)offset_bottom, (int)offset_top);
@@ -1114,8 +1164,7 @@ int ugks_UGKS_update(ugks_block *blk, void *stream, void *Fin, const void *flux_
                      const void *area, const void *macro, const void *gQ, const void *residual, double dt, void *flag) {
     UGKS_BLK_CHECK(blk);
     const Consts &C = blk->c;
-    long long nw = (long long)C.ni * C.nj * nvchunks(C);
-    hipLaunchKernelGGL(k_UGKS_update, dim3(blocks_for_waves(nw)), dim3(kBlock), 0, S(stream), C, (d2 *)Fin,
+    hipLaunchKernelGGL(k_UGKS_update, dim3(blocks_for_waves((long long)C.ni * C.nj)), dim3(kBlock), 0, S(stream), C, (d2 *)Fin,
                        (const d2 *)flux_f_S, (const d2 *)flux_f_W, (const double *)area, (const d4 *)macro,
                        (const d2 *)gQ, (const d4 *)residual, dt, (double *)flag);
     return ugks_rt::check_launch("UGKS_update");
