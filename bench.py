@@ -63,6 +63,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=5.0, help="target length of one CPU sample run")
     ap.add_argument("--profile-steps", type=int, default=5, help="steps timed per kernel with HIP events")
+    ap.add_argument("--per-kernel-steps", type=int, default=0, metavar="N",
+                    help="also time N FP64 steps of the reference-shaped per-kernel path (UGKSim.one_step)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run the plain copy kernel on MIB MiB (byte-counter calibration) and exit")
     return ap.parse_args()
@@ -227,6 +229,31 @@ def run_precision(gs, precision, a, upd_per_step, workload):
     return rep, final
 
 
+def per_kernel_path(gs, steps, upd_per_step):
+    """The reference's call sequence (UGKSim.one_step: one launch per reference
+    kernel and block, with its host readbacks of the flag and time step), in
+    FP64, case by case: 2 untimed steps, then `steps` timed ones; the times of
+    the cases add up to one step of the workload."""
+    from pyugks_amd import lib as L
+    from pyugks_amd.sim import UGKSim
+    total = 0.0
+    for g in gs:
+        sim = UGKSim(copy.deepcopy(g))
+        for _ in range(2):
+            sim.one_step(True)
+        L.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            sim.one_step(True)
+        L.synchronize()
+        total += (time.perf_counter() - t0) / steps
+        for b in sim.blocks:
+            b.free_device()
+    print(f"[bench] per-kernel path: {1e3 * total:.3f} ms per step", file=sys.stderr, flush=True)
+    return {"ms_per_step": 1e3 * total, "value": upd_per_step / total, "unit": "cell-velocity updates/s",
+            "steps": steps, "dtype": "f64", "path": "per-kernel (UGKSim.one_step)"}
+
+
 def errors_between(final_a, final_b):
     from pyugks_amd.metrics import field_errors
     worst = {}
@@ -374,6 +401,8 @@ def main():
                        "roofline": f["roofline"], "mass_drift": f["mass_drift"],
                        "kernel_ms_per_launch": f["kernel_ms_per_launch"], "flag": f["flag"]}
     finals.clear()
+    if a.per_kernel_steps > 0:
+        out["per_kernel_path"] = per_kernel_path(gs, a.per_kernel_steps, upd_per_step)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(gs, a, workload)
         out["cpu_baseline"]["gpu_over_cpu"] = r["value"] / out["cpu_baseline"]["value"]
