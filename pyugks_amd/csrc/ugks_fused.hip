@@ -1188,7 +1188,8 @@ __global__ void __launch_bounds__(kT) k_update(const FBlock *__restrict__ blocks
     const double rA = 1.0 / A, rden = 1.0 / (1.0 + 0.5 * dt / tau);
     const double htn = (0.5 * dt) / tau, hto = (0.5 * dt) / tau_old;
     // the per-cell constants of the velocity loop in scalar registers: the FP32
-    // update then needs 72 VGPRs (seven waves per SIMD) instead of 104, the FP64 one 82
+    // update then needs 74 VGPRs (six waves per SIMD) instead of 104, the FP64 one 84
+    // (five); held to 72 for seven waves, the FP32 one spills and runs slower
     const double po1 = uniform_d(pold.s1), po2 = uniform_d(pold.s2), po3 = uniform_d(pold.s3);
     const double pn1 = uniform_d(p.s1), pn2 = uniform_d(p.s2), pn3 = uniform_d(p.s3);
     const double Qx = uniform_d(Q.x), Qy = uniform_d(Q.y);
