@@ -32,6 +32,8 @@ def small_cases():
     out["cavity_weno_rho_kn1"] = cases.cavity(10, ("NC", 16), "WENO5", 1.0, rho_perturb=0.01)
     # 33 x 33 velocities: several 256-velocity chunks per wall face, and a 17 x 64 + 1 velocity loop
     out["cavity_weno_nv1089"] = cases.cavity(6, ("NC", 32), "WENO5", 0.01, lid=0.2, n_cells_j=5)
+    # 6 x 6 velocities: fewer than one wavefront's 64 lanes, so most lanes of every velocity loop idle
+    out["cavity_weno_gh36"] = cases.cavity(8, ("GH", 6), "WENO5", 0.05, lid=0.2, n_cells_j=6)
     # a block two cells high between two walls
     out["thin_two_rows"] = cases.cavity(9, ("NC", 8), "vanLeer", 0.05, lid=0.2, n_cells_j=2)
     out["channel_adsorbing"] = cases.cfg5_channel(widths=[6, 10, 8], height=8, nc=8, Kn=0.05)
