@@ -2,10 +2,12 @@
 // time step, behind the reference's kernel names and argument order.
 //
 // Mapping: the reference runs one work-item per (cell or face) with a serial
-// loop over velocities.  Here a 64-lane wavefront owns one (cell/face,
-// 64-velocity chunk) for per-velocity work, or one cell/face for the
-// velocity reductions (lanes stride over v, then a shuffle tree).  The f
-// layout [i][j][v] of (g,h) pairs makes every wave access a contiguous
+// loop over velocities.  Here a 64-lane wavefront owns one cell or face and
+// its lanes stride over the velocities: the heavy per-velocity kernels
+// (iFace, distFlux, UGKS_update) form the cell's or face's set-up once and
+// loop, the velocity reductions end in a shuffle tree; the light ones (ghost
+// fills, initial state) take one (cell, 64-velocity chunk) per wavefront.
+// The f layout [i][j][v] of (g,h) pairs makes every wave access a contiguous
 // 1 KiB run, one dwordx4 per lane.
 #include "ugks_device.hpp"
 #include "ugks_runtime.hpp"
