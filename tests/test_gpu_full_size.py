@@ -3,6 +3,8 @@
 - The 64^2 Gauss-Hermite vanLeer cavity (the reference's own CPU case):
   twenty FP64 steps against the oracle at 1e-12 per step, and the mixed
   plan within 1e-5 of it.
+- The 2x2x128^2 cavity with 33^2 velocities through the per-kernel path:
+  three steps against the oracle at 1e-12 per step.
 - The 64-case batch, the 512^2 cavity and the adsorbing channel are too
   large for the oracle to follow in a test, so they are checked through
   properties that do not depend on size: the mixed plan within 1e-5 (rho,
@@ -81,6 +83,31 @@ def test_cavity_64_gh24_vanleer_fp64_and_mixed_against_the_oracle(hip):
     for k in ("rho_linf", "U_linf", "T_linf"):
         assert e[k] <= 1e-5, (k, e[k])
     assert mixed["flag"] == 0.0
+
+
+def test_cavity_2x2x128_per_kernel_path_against_the_oracle(hip):
+    """The reference-shaped per-kernel path (UGKSim.one_step) at the 2x2x128^2
+    size with 33^2 velocities: four blocks of 128^2 cells, 1089 velocities
+    (17 full wavefront rounds and one lane of an 18th per face or cell), the
+    oracle's f, macro fields and heat flux to 1e-12 after each of 3 steps."""
+    from oracle import OracleSim
+    from pyugks_amd.sim import UGKSim
+    g = cases.cfg2_cavity_2x2()
+    print("[gpu] full size: 2x2x128^2 cavity, 33^2 velocities, per-kernel path, 3 steps", flush=True)
+    ora = OracleSim(copy.deepcopy(g), threads=_threads())
+    gpu = UGKSim(copy.deepcopy(g))
+    for step in range(3):
+        gpu.one_step(True)
+        ora.one_step(True)
+        assert abs(gpu.gdata.dt - ora.dt) <= TOL * ora.dt, step
+        for bg, bo in zip(gpu.blocks, ora.blocks):
+            assert rel_err(bg.macro_D.download(), bo["macro"]) <= TOL, step
+            assert rel_err(bg.f_D.download(), bo["f"]) <= TOL, step
+    for bg, bo in zip(gpu.blocks, ora.blocks):
+        assert rel_err(bg.Q_D.download(), bo["Q"], SCALE_FLOOR["Q"]) <= TOL
+        bg.free_device()
+    del gpu, ora
+    gc.collect()
 
 
 def test_batch_of_64_cases_mixed_tracks_fp64_and_conserves_mass(hip):
