@@ -437,65 +437,78 @@ __device__ __forceinline__ WallSite wall_site(const Consts &C, int face, int ci)
 }
 __host__ __device__ inline int wall_count(int ni, int nj, int face) { return (face == GNORTH || face == GSOUTH) ? ni : nj; }
 
+// The wall kernels take one 256-thread workgroup per wall face (a launch has
+// only as many faces as the wall has cells, so one wavefront each would leave
+// most of the chip idle); its threads stride over the velocities, and these
+// sums end over the four wavefronts: every thread gets the totals.
+template <int N> __device__ __forceinline__ void wall_face_sum(double (&a)[N], double (*red)[N]) {
+#pragma unroll
+    for (int q = 0; q < N; ++q) a[q] = wave_sum(a[q]);
+    if (lane_id() == 0) {
+#pragma unroll
+        for (int q = 0; q < N; ++q) red[threadIdx.x >> 6][q] = a[q];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < N; ++q) a[q] = (red[0][q] + red[1][q]) + (red[2][q] + red[3][q]);
+}
+
 // accommodatingWallDist (clUGKS.c:776-906): diffuse re-emission with zero net mass flux
-__global__ void k_accommodatingWallDist(Consts C, const d2 *normal, int face, const d4 *wall_prop, d2 *flux_f) {
-    long long w = wave_id();
-    if (w >= wall_count(C.ni, C.nj, face)) return;
-    WallSite s = wall_site(C, face, (int)w);
-    int lane = lane_id();
+__global__ void __launch_bounds__(kBlock) k_accommodatingWallDist(Consts C, const d2 *normal, int face,
+                                                                  const d4 *wall_prop, d2 *flux_f) {
+    __shared__ double red[kWaves][2];
+    WallSite s = wall_site(C, face, (int)blockIdx.x);
     d2 n = normal[sidx(C, s.gi, s.gj, s.fid)];
     d4 wall = wall_prop[(size_t)s.ci * 4 + face];
     d2 wl = to_local(mk2(wall.s1, wall.s2), n);
     wall.s1 = wl.x; wall.s2 = wl.y;
-    double out = 0, in = 0;
-    for (int v = lane; v < C.NV; v += 64) {
+    double io[2] = {0.0, 0.0};   // outgoing (towards the wall) and re-emitted mass flux
+    for (int v = threadIdx.x; v < C.NV; v += kBlock) {
         d2 uv = face_velocity(C, v, n);
-        if (leaves_wall(uv.x, s.rot)) in -= uv.x * maxwell(C, wall, uv, v).x;
-        else out += uv.x * flux_f[fidx(C, s.gi, s.gj, v)].x;
+        if (leaves_wall(uv.x, s.rot)) io[1] -= uv.x * maxwell(C, wall, uv, v).x;
+        else io[0] += uv.x * flux_f[fidx(C, s.gi, s.gj, v)].x;
     }
-    double ratio = wave_sum(out) / wave_sum(in);
-    for (int v = lane; v < C.NV; v += 64) {
+    wall_face_sum<2>(io, red);
+    const double ratio = io[0] / io[1];
+    for (int v = threadIdx.x; v < C.NV; v += kBlock) {
         d2 uv = face_velocity(C, v, n);
         if (leaves_wall(uv.x, s.rot)) flux_f[fidx(C, s.gi, s.gj, v)] = ratio * maxwell(C, wall, uv, v);
     }
 }
 
 // wallFlux (clUGKS.c:910-1011): first-order wall flux u f len dt and its moments
-__global__ void k_wallFlux(Consts C, const d2 *normal, const double *side_length, int face, d2 *flux_f, d4 *flux_macro,
-                           double dt) {
-    long long w = wave_id();
-    if (w >= wall_count(C.ni, C.nj, face)) return;
-    WallSite s = wall_site(C, face, (int)w);
-    int lane = lane_id();
+__global__ void __launch_bounds__(kBlock) k_wallFlux(Consts C, const d2 *normal, const double *side_length, int face,
+                                                     d2 *flux_f, d4 *flux_macro, double dt) {
+    __shared__ double red[kWaves][4];
+    WallSite s = wall_site(C, face, (int)blockIdx.x);
     d2 n = normal[sidx(C, s.gi, s.gj, s.fid)];
     double L = side_length[sidx(C, s.gi, s.gj, s.fid)];
-    double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
-    for (int v = lane; v < C.NV; v += 64) {
+    double m[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int v = threadIdx.x; v < C.NV; v += kBlock) {
         d2 uv = face_velocity(C, v, n);
         size_t fi = fidx(C, s.gi, s.gj, v);
         d2 wd = flux_f[fi];
         flux_f[fi] = ((uv.x * wd) * L) * dt;
-        s0 += uv.x * wd.x;
-        s1 += uv.x * uv.x * wd.x;
-        s2 += uv.x * uv.y * wd.x;
-        s3 += 0.5 * uv.x * ((uv.x * uv.x + uv.y * uv.y) * wd.x + wd.y);
+        m[0] += uv.x * wd.x;
+        m[1] += uv.x * uv.x * wd.x;
+        m[2] += uv.x * uv.y * wd.x;
+        m[3] += 0.5 * uv.x * ((uv.x * uv.x + uv.y * uv.y) * wd.x + wd.y);
     }
-    s0 = wave_sum(s0); s1 = wave_sum(s1); s2 = wave_sum(s2); s3 = wave_sum(s3);
-    if (lane == 0) {
-        d2 g = to_global(mk2(s1, s2), n);
-        flux_macro[cidx(C, s.gi, s.gj)] = (dt * L) * mk4(s0, g.x, g.y, s3);
+    wall_face_sum<4>(m, red);
+    if (threadIdx.x == 0) {
+        d2 g = to_global(mk2(m[1], m[2]), n);
+        flux_macro[cidx(C, s.gi, s.gj)] = (dt * L) * mk4(m[0], g.x, g.y, m[3]);
     }
 }
 
 // wallMassEnergyFluxes (clUGKS.c:1019-1147): in/out mass, momentum, energy per wall face
-__global__ void k_wallMassEnergyFluxes(Consts C, const d2 *normal, int face, const d2 *flux_f, double *wall_flux) {
-    long long w = wave_id();
-    if (w >= wall_count(C.ni, C.nj, face)) return;
-    WallSite s = wall_site(C, face, (int)w);
-    int lane = lane_id();
+__global__ void __launch_bounds__(kBlock) k_wallMassEnergyFluxes(Consts C, const d2 *normal, int face,
+                                                                 const d2 *flux_f, double *wall_flux) {
+    __shared__ double red[kWaves][8];
+    WallSite s = wall_site(C, face, (int)blockIdx.x);
     d2 n = normal[sidx(C, s.gi, s.gj, s.fid)];
     double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};   // in0, out0, in1, out1, ...
-    for (int v = lane; v < C.NV; v += 64) {
+    for (int v = threadIdx.x; v < C.NV; v += kBlock) {
         d2 uv = face_velocity(C, v, n);
         d2 wd = flux_f[fidx(C, s.gi, s.gj, v)];
         int o = leaves_wall(uv.x, s.rot);
@@ -505,9 +518,8 @@ __global__ void k_wallMassEnergyFluxes(Consts C, const d2 *normal, int face, con
         a[4 + o] += uv.y * wd.x;
         a[6 + o] += e;
     }
-#pragma unroll
-    for (int q = 0; q < 8; ++q) a[q] = wave_sum(a[q]);
-    if (lane == 0) {
+    wall_face_sum<8>(a, red);
+    if (threadIdx.x == 0) {
         double *o = wall_flux + ((size_t)s.ci * 4 + face) * 8;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
@@ -1087,7 +1099,7 @@ int ugks_accommodatingWallDist(ugks_block *blk, void *stream, const void *normal
     UGKS_BLK_CHECK(blk);
     UGKS_WALL_CHECK(face);
     const Consts &C = blk->c;
-    hipLaunchKernelGGL(k_accommodatingWallDist, dim3(blocks_for_waves(wall_count(C.ni, C.nj, face))), dim3(kBlock), 0,
+    hipLaunchKernelGGL(k_accommodatingWallDist, dim3((unsigned)wall_count(C.ni, C.nj, face)), dim3(kBlock), 0,
                        S(stream), C, (const d2 *)normal, (int)face, (const d4 *)wall_prop, (d2 *)flux_f);
     return ugks_rt::check_launch("accommodatingWallDist");
 }
@@ -1097,7 +1109,7 @@ int ugks_wallFlux(ugks_block *blk, void *stream, const void *normal, const void 
     UGKS_BLK_CHECK(blk);
     UGKS_WALL_CHECK(face);
     const Consts &C = blk->c;
-    hipLaunchKernelGGL(k_wallFlux, dim3(blocks_for_waves(wall_count(C.ni, C.nj, face))), dim3(kBlock), 0, S(stream), C,
+    hipLaunchKernelGGL(k_wallFlux, dim3((unsigned)wall_count(C.ni, C.nj, face)), dim3(kBlock), 0, S(stream), C,
                        (const d2 *)normal, (const double *)side_length, (int)face, (d2 *)flux_f, (d4 *)flux_macro, dt);
     return ugks_rt::check_launch("wallFlux");
 }
@@ -1108,7 +1120,7 @@ int ugks_wallMassEnergyFluxes(ugks_block *blk, void *stream, const void *normal,
     UGKS_BLK_CHECK(blk);
     UGKS_WALL_CHECK(face);
     const Consts &C = blk->c;
-    hipLaunchKernelGGL(k_wallMassEnergyFluxes, dim3(blocks_for_waves(wall_count(C.ni, C.nj, face))), dim3(kBlock), 0,
+    hipLaunchKernelGGL(k_wallMassEnergyFluxes, dim3((unsigned)wall_count(C.ni, C.nj, face)), dim3(kBlock), 0,
                        S(stream), C, (const d2 *)normal, (int)face, (const d2 *)flux_f, (double *)wall_flux);
     return ugks_rt::check_launch("wallMassEnergyFluxes");
 }
